@@ -1,0 +1,281 @@
+#!/usr/bin/env python
+"""ERI + dERI shell-quartet throughput on one MI355X.
+
+Workload (default): the 10-atom synthetic cluster (6 C + 4 H, seeded, 8-bohr
+cube, minimum separation 1.8 bohr), even-tempered spherical basis (280 AO /
+300 Cartesian functions), Schwarz threshold 1e-12, FP64.  One step is one
+pass of the hot path over the whole molecule:
+
+  * the ERI pass: every canonical screened shell quartet, contracted with the
+    density into J and K (integral-direct Fock build), and
+  * the dERI pass: every canonical screened quartet not on a single atom,
+    differentiated and contracted with the density pair products (adjoint
+    HF gradient, two-electron part).
+
+The density is the converged RHF density of the same molecule (SCF run
+before the timed region).  value = (ERI quartets + dERI quartets) / step
+time, inputs resident in HBM.  The HF energy + gradient wall time is reported
+beside it.
+
+Prints ONE JSON line (rank 0).  --gpus above 1 is refused: this build runs on
+one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "ERI+dERI shell-quartets/s; HF-gradient wall time, mixed vs FP64 @1e-8 Eh"
+PEAK_FP64 = 78.6e12     # MI355X FP64 (vector and matrix) peak, FLOP/s
+PEAK_HBM = 8.0e12       # bytes/s
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def kernel_table(eng):
+    """Per profile slot: name, algorithmic flops per primitive quartet
+    (including the epilogue contraction)."""
+    import ctypes
+    L = eng.L
+    nc = L.qx_num_classes()
+    info = {}
+    for k in range(nc):
+        l4 = (ctypes.c_int * 4)()
+        npart = ctypes.c_int()
+        L.qx_class_info(k, l4, ctypes.byref(npart))
+        ef = ctypes.c_double()
+        pf = (ctypes.c_double * 12)()
+        pc = (ctypes.c_int * 12)()
+        L.qx_class_flops(k, ctypes.byref(ef), pf, pc)
+        la, lb, lc, ld = l4[:]
+        ncart = lambda l: (l + 1) * (l + 2) // 2
+        N = ncart(la) * ncart(lb) * ncart(lc) * ncart(ld)
+        nm = f"{la}{lb}{lc}{ld}"
+        # J/K epilogue: 6 fused multiply-adds per integral
+        info[k] = (f"qx_k_e{nm}", ef.value + 12.0 * N)
+        for p in range(npart.value):
+            # adjoint epilogue: density pair product (8 flops) + 2 per block
+            info[nc + 12 * k + p] = (f"qx_k_d{nm}_{p}", pf[p] + 8.0 * N + 2.0 * pc[p] * N)
+    return info
+
+
+def cpu_baseline(mol, basis, tol, eng, seconds):
+    """Times the CPU oracle (C restatement of the reference's quartet loop
+    and of the Libint HGP arithmetic) on a seeded uniform sample of the same
+    canonical screened quartets, ERI and first-derivative, with OpenMP."""
+    import numpy as np
+    import oracle
+    ob = oracle.OracleBasis(mol, basis)
+    vals, pairs = eng.schwarz_screening()
+    l = np.array([s.l for s in basis.shells])
+    atom = np.array([s.atom for s in basis.shells])
+    # canonical pair order as the device builds it: class, then Schwarz desc
+    p = pairs
+    la, lb = l[p[:, 0]], l[p[:, 1]]
+    a = np.where(la >= lb, p[:, 0], p[:, 1])
+    b = np.where(la >= lb, p[:, 1], p[:, 0])
+    cls = l[a] * (l[a] + 1) // 2 + l[b]
+    Q = vals[p[:, 0], p[:, 1]]
+    order = np.lexsort((-Q, cls))
+    a, b, cls, Q = a[order], b[order], cls[order], Q[order]
+    rng = np.random.default_rng(11)
+    # sample bra pairs proportionally to their admissible ket counts
+    tol2 = tol * tol
+    counts = np.zeros(len(a), dtype=np.int64)
+    ket_lists = {}
+    for X in np.unique(cls):
+        for Y in np.unique(cls):
+            if Y > X:
+                continue
+            bi = np.where(cls == X)[0]
+            ki = np.where(cls == Y)[0]
+            kq = Q[ki]
+            for ii, i in enumerate(bi):
+                kc = int(np.searchsorted(-kq, -tol2 / Q[i], side="right")) if Q[i] > 0 else 0
+                if X == Y:
+                    kc = min(kc, ii + 1)
+                counts[i] += kc
+                ket_lists.setdefault(i, []).append((ki, kc))
+    total = counts.sum()
+    nthreads = min(16, os.cpu_count() or 1)
+
+    cum = np.cumsum(counts)
+
+    def sample(n, deriv):
+        out = []
+        while len(out) < n:
+            t = int(rng.integers(total))
+            i = int(np.searchsorted(cum, t, side="right"))
+            r = t - (cum[i - 1] if i > 0 else 0)
+            for ki, kc in ket_lists[i]:
+                if r < kc:
+                    j = ki[r]
+                    break
+                r -= kc
+            q = (a[i], b[i], a[j], b[j])
+            if deriv and len({atom[s] for s in q}) == 1:
+                continue
+            out.append(q)
+        return np.array(out, dtype=np.int32)
+
+    res = {}
+    for deriv in (False, True):
+        n = 2000
+        while True:
+            qs = sample(n, deriv)
+            t0 = time.perf_counter()
+            ob.quartet_batch(qs, nthreads=nthreads, deriv=deriv)
+            dt = time.perf_counter() - t0
+            if dt > seconds / 2 or n >= 2_000_000:
+                break
+            n = int(n * min(20.0, max(2.0, (seconds / 2) / max(dt, 1e-3))))
+        res["deri" if deriv else "eri"] = (len(qs) / dt, len(qs), dt)
+    return res, nthreads
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU baseline budget (0 = skip)")
+    ap.add_argument("--workload", default="cluster10", choices=["cluster10", "cluster24", "water"])
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is design-only)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from quax_amd import workloads
+    from quax_amd.basis import BasisSet
+    from quax_amd.engine import EriEngine
+    from quax_amd.methods.hartree_fock import restricted_hartree_fock, rhf_gradient
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(1)
+    tol = 1e-12
+    if args.workload == "water":
+        mol, bname, pure = workloads.water_sto3g(), "sto-3g", False
+    elif args.workload == "cluster24":
+        mol, bname, pure = workloads.cluster24(), "even-tempered", True
+    else:
+        mol, bname, pure = workloads.cluster10(), "even-tempered", True
+    basis = BasisSet.build(mol, bname, pure=pure)
+    t_init = time.perf_counter()
+    eng = EriEngine(mol, basis, ints_tol=tol)
+    torch.cuda.synchronize()
+    t_init = time.perf_counter() - t_init
+    cq = eng.class_quartets()
+    q_eri = int(sum(x[1] for x in cq))
+    q_deri = int(sum(x[2] for x in cq))
+    log(f"[bench] {args.workload}: nbf {eng.nbf} (cart {eng.ncart}), ERI quartets {q_eri}, dERI quartets {q_deri}, "
+        f"init {t_init:.2f} s")
+
+    # HF energy + gradient (FP64), wall time end to end after set-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    E, aux = restricted_hartree_fock(eng, mol.nelectron(), {"diis": True, "spectral_shift": False, "conv": 1e-10,
+                                                            "maxit": 200}, return_aux_data=True)
+    torch.cuda.synchronize()
+    t_scf = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    grad = rhf_gradient(eng, aux)
+    torch.cuda.synchronize()
+    t_grad = time.perf_counter() - t0
+    log(f"[bench] RHF E = {E:.10f} in {aux['iterations']} iterations ({t_scf:.2f} s), gradient {t_grad:.3f} s, "
+        f"|sum g| = {abs(grad.reshape(-1, 3).sum(0)).max():.2e}")
+    D = aux["D"].contiguous()
+    J = torch.empty_like(D)
+    K = torch.empty_like(D)
+
+    def step():
+        eng.jk(D, out=(J, K))
+        eng.eri_grad(D)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    eng.profile(True)
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step()
+        log(f"[bench] step {i + 1}/{args.steps}")
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms, launches, pq = eng.profile_read()
+    eng.profile(False)
+    ms_step = dt * 1e3 / args.steps
+    value = (q_eri + q_deri) / (dt / args.steps)
+
+    # dominant kernel and its roofline (FP64 compute bound)
+    table = kernel_table(eng)
+    nc = eng.L.qx_num_classes()
+    eri_ms = float(ms[:nc].sum()) / args.steps
+    deri_ms = float(ms[nc:].sum()) / args.steps
+    slot = int(np.argmax(ms))
+    name, fl = table[slot]
+    avg_ms = ms[slot] / launches[slot]
+    flops_per_launch = fl * pq[slot] / launches[slot]
+    achieved = flops_per_launch / (avg_ms * 1e-3)
+    total_flops = sum(table[s][1] * pq[s] for s in range(len(ms)) if launches[s]) / args.steps
+    kernels = sorted(((float(ms[s]) / args.steps, table[s][0], int(pq[s] // max(launches[s], 1)))
+                      for s in range(len(ms)) if launches[s]), reverse=True)[:8]
+
+    cpu = None
+    if args.cpu_seconds > 0:
+        try:
+            res, nthreads = cpu_baseline(mol, basis, tol, eng, args.cpu_seconds)
+            r_e, n_e, t_e = res["eri"]
+            r_d, n_d, t_d = res["deri"]
+            t_cpu_step = q_eri / r_e + q_deri / r_d
+            cpu = {"value": (q_eri + q_deri) / t_cpu_step, "unit": "shell-quartets/s", "cores": nthreads,
+                   "kind": "port",
+                   "sample": f"{n_e} ERI + {n_d} dERI canonical screened quartets drawn uniformly from the same "
+                             f"workload, CPU oracle (C, OpenMP {nthreads} threads): ERI {r_e:.3g}/s, dERI "
+                             f"{r_d:.3g}/s, combined over the step's mix"}
+        except Exception as e:  # the baseline is reported, never required
+            log(f"[bench] cpu baseline failed: {e}")
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "shell-quartets/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{args.workload}: {mol.natom()} atoms, {bname} "
+                               f"{'spherical' if pure else 'Cartesian'} basis ({eng.nbf} AO / {eng.ncart} Cartesian), "
+                               f"Schwarz 1e-12; step = direct J/K ERI pass + adjoint dERI gradient pass",
+                   "eri_quartets": q_eri, "deri_quartets": q_deri, "eri_ms": eri_ms, "deri_ms": deri_ms},
+        "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64, "traffic": None, "kernel": name, "avg_launch_ms": avg_ms,
+                     "flops_per_launch": flops_per_launch, "step_flops": total_flops,
+                     "step_tflops": total_flops / (ms_step * 1e-3) / 1e12},
+        "cpu_baseline": cpu,
+        "hf": {"energy": E, "scf_iterations": aux["iterations"], "scf_wall_s": t_scf, "grad_wall_s": t_grad,
+               "hf_gradient_wall_s_fp64": t_scf + t_grad, "hf_gradient_wall_s_mixed": None},
+        "top_kernels_ms_per_step": kernels,
+    }
+    print(json.dumps(out), flush=True)
+    eng.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,155 @@
+/*
+ * quax_amd.h -- C ABI of the MI355X-native two-electron integral library
+ * (libquax_hip.so).  It replaces the pybind11 module ``libint_interface`` of
+ * Quax for the ERI path (reference: quax/integrals/libint_interface.cc) and adds
+ * the integral-direct consumers the RHF energy and gradient need.
+ *
+ * Conventions
+ *  - Every entry point returns an int status: QX_OK (0) or one of the QX_E*
+ *    codes; nothing aborts.  qx_last_error() gives the message.  The reference
+ *    aborts on a bad deriv_vec length (libint_interface.cc:1098) and throws
+ *    std::invalid_argument for an unknown operator (:520); both become
+ *    QX_EINVAL here.
+ *  - A context holds one molecule and one basis (the reference keeps them in
+ *    module globals, libint_interface.cc:24-35).  Contexts are independent; one
+ *    context must not be used from two threads at once.
+ *  - Output and density buffers are DEVICE pointers owned by the caller, on the
+ *    context's device; ``stream`` is a hipStream_t (NULL = default stream).
+ *    Calls are asynchronous with respect to the host unless stated.
+ *  - Tensors use the AO basis of the context: Cartesian when pure == 0 (the
+ *    reference forces Cartesian shells, libint_interface.cc:49), spherical
+ *    (l >= 2 shells as 2l+1 real solid harmonics, m = -l..l) when pure == 1.
+ *  - Layouts follow the reference: ERI tensor row-major (p,q,r,s) in chemists'
+ *    notation (libint_interface.cc:570-585); derivative tensor
+ *    (n_unique, p, q, r, s) with the nuclear index outermost (:2653).
+ */
+#ifndef QUAX_AMD_H
+#define QUAX_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct qx_ctx qx_ctx;
+
+enum {
+  QX_OK = 0,
+  QX_EINVAL = 1,        /* bad argument */
+  QX_ENOMEM = 2,        /* device allocation failed */
+  QX_EHIP = 3,          /* HIP runtime error */
+  QX_EUNSUPPORTED = 4,  /* valid request outside what this build implements */
+  QX_ENODEVICE = 5      /* no usable GPU */
+};
+
+enum { QX_PREC_FP64 = 0, QX_PREC_MIXED = 1 };
+
+/* Replaces initialize(xyzfilename, basis1, basis2, basis3, basis4, ints_tol)
+ * (libint_interface.cc:38-95).  Takes the geometry in bohr and the shells
+ * directly instead of an xyz file and basis names; raw contraction
+ * coefficients are normalised the way Libint's Shell::renorm does.
+ * Computes the Schwarz table (libint_interface.cc:158-229) on the device and
+ * builds the screened, class-sorted quartet lists.  ints_tol is the
+ * reference's ints_tolerance (quartets with Q_ab*Q_cd < tol^2 are skipped). */
+int qx_initialize(int natom, const double* xyz_bohr, const double* nuclear_charges, int nshell,
+                  const int* shell_l, const int* shell_atom, const int* shell_nprim, const double* exps,
+                  const double* coefs, int pure, double ints_tol, int device, qx_ctx** out);
+
+/* Replaces finalize() (libint_interface.cc:97-99). */
+int qx_finalize(qx_ctx* ctx);
+
+const char* qx_last_error(const qx_ctx* ctx);
+
+/* Basis sizes: AO functions (spherical count when pure) and Cartesian. */
+int qx_nbf(const qx_ctx* ctx, int* nbf, int* ncart);
+
+/* schwarz_screening (libint_interface.cc:158-229), host outputs:
+ * values[nshell*nshell] = max_f |(f1 f2|f1 f2)| (Cartesian, symmetric);
+ * pairs[2*k] = (s1, s2), s2 <= s1, kept by Q >= tol^2/max Q, reference order.
+ * pairs needs room for nshell*(nshell+1) ints. */
+int qx_schwarz_screening(qx_ctx* ctx, double* values, int* pairs, int* npairs);
+
+/* Number of canonical (8-fold unique) screened shell quartets. */
+int qx_num_quartets(const qx_ctx* ctx, long long* nquartets);
+
+/* compute_2e_int("eri", 0.) (libint_interface.cc:486-647): full nbf^4 tensor
+ * written to out (device, nbf^4 doubles).  Screened quartets are exact zeros. */
+int qx_compute_2e_int(qx_ctx* ctx, double* out, void* stream);
+
+/* eri_deriv_core(deriv_order) (libint_interface.cc:2582-2779): out holds
+ * how_many_derivs(natom, order) * nbf^4 doubles.  deriv_order 1 only
+ * (QX_EUNSUPPORTED otherwise).  Quartets on a single atom are skipped. */
+int qx_eri_deriv_core(qx_ctx* ctx, int deriv_order, double* out, void* stream);
+
+/* compute_2e_deriv("eri", 0., deriv_vec) (libint_interface.cc:1097-1335):
+ * one nbf^4 slice.  len must equal 3*natom (QX_EINVAL otherwise); total order
+ * 0 or 1 (QX_EUNSUPPORTED above). */
+int qx_compute_2e_deriv(qx_ctx* ctx, const int* deriv_vec, int len, double* out, void* stream);
+
+/* how_many_derivs (utils.h:131-140): C(3*natom + order - 1, order). */
+long long qx_how_many_derivs(int natom, int deriv_order);
+
+/* generate_multi_index_lookup (utils.h:142-172): combinations with
+ * replacement of range(nparams), lexicographic, written row-major into
+ * out[count*deriv_order] (host).  Returns the count, or -1 if cap is short. */
+long long qx_generate_multi_index_lookup(int nparams, int deriv_order, int* out, long long cap);
+
+/* Integral-direct J and K (hartree_fock.py:25-28, 79-80 contract the stored
+ * tensor the same way): J_pq = sum_rs (pq|rs) D_rs, K_pq = sum_rs (pr|qs) D_rs.
+ * D, J, K: nbf x nbf device buffers.  precision QX_PREC_MIXED evaluates
+ * quartets whose Schwarz bound sqrt(Q_ab Q_cd) is below fp32_threshold with
+ * an FP32 recursion, summed in FP64. */
+int qx_jk_build(qx_ctx* ctx, const double* D, double* J, double* K, int precision, double fp32_threshold,
+                void* stream);
+
+/* Adjoint two-electron gradient (no reference counterpart: the reference
+ * differentiates forward through eri_deriv_core, tei.py:25-34):
+ * grad[3*natom] += sum_pqrs (2 D_pq D_rs - D_pr D_qs) d(pq|rs)/dR.
+ * Integrals are recomputed, the derivative tensor is never stored. */
+int qx_eri_grad(qx_ctx* ctx, const double* D, double* grad, int precision, double fp32_threshold, void* stream);
+
+/* One-electron integrals S, T, V (nbf x nbf device buffers); Libint
+ * overlap / kinetic / nuclear operators as compute_1e_int uses them
+ * (libint_interface.cc:232-299). */
+int qx_oei(qx_ctx* ctx, double* S, double* T, double* V, void* stream);
+
+/* One-electron gradient contraction (oei_deriv_core, libint_interface.cc:
+ * 2403-2579, contracted on the fly): grad[3*natom] +=
+ * sum_pq Pd_pq d(T+V)_pq/dR + Wd_pq dS_pq/dR.  Pd, Wd: nbf x nbf. */
+int qx_oei_grad(qx_ctx* ctx, const double* Pd, const double* Wd, double* grad, void* stream);
+
+/* One-electron derivative tensors (oei_deriv_core(1)): dS, dT, dV each
+ * 3*natom * nbf * nbf doubles on the device. */
+int qx_oei_deriv(qx_ctx* ctx, double* dS, double* dT, double* dV, void* stream);
+
+/* Materialise the 8-fold unique Cartesian integrals in the class-blocked
+ * layout: for every quartet list, element (component k, quartet t) at
+ * base + k * nquart + t.  Requires single-primitive shells; out must hold
+ * qx_store_size() doubles. */
+int qx_store_size(const qx_ctx* ctx, long long* ndoubles);
+int qx_eri_store(qx_ctx* ctx, double* out, void* stream);
+
+/* Measurement support (no reference counterpart).
+ * qx_class_quartets: canonical screened quartets per kernel class (eri[]) and
+ * those not on a single atom (deri[], the derivative work); arrays of
+ * qx_num_classes() entries.  qx_class_info: angular momenta (l[4]) and the
+ * number of derivative kernel parts of class cls.
+ * qx_profile_enable(1) resets and starts recording a HIP event pair around
+ * every kernel launch; qx_profile_read synchronises and returns, per slot
+ * (ERI class c -> c; derivative part k of class c -> num_classes + 12c + k),
+ * the summed kernel time in ms, the launch count and the primitive-pair
+ * quartets processed.  Arrays hold qx_profile_slots() entries. */
+int qx_num_classes(void);
+int qx_profile_slots(void);
+int qx_class_info(int cls, int* l4, int* nparts);
+int qx_class_quartets(const qx_ctx* ctx, long long* eri, long long* deri);
+/* Algorithmic FP operations (mul/add 1, fma 2) of one primitive-quartet ERI
+ * body of class cls and of each derivative part, epilogue excluded, and the
+ * number of derivative blocks each part produces. */
+int qx_class_flops(int cls, double* eri, double* parts, int* ncomb);
+int qx_profile_enable(qx_ctx* ctx, int on);
+int qx_profile_read(qx_ctx* ctx, double* ms, long long* launches, long long* pquartets);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

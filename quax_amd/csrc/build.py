@@ -1,0 +1,105 @@
+"""Builds libquax_hip.so for gfx950.
+
+Steps: generate the per-class kernel sources (gen_eri.py) into csrc/_gen,
+compile every translation unit with hipcc in parallel, link the shared
+library into quax_amd/lib/.  Objects are cached under csrc/_build keyed by a
+hash of (source, headers, flags); ``force=True`` recompiles everything.
+
+Usage: python -m quax_amd.csrc.build [--force] [--jobs N] [--lmax L] [--fp32]
+"""
+from __future__ import annotations
+
+import argparse
+import concurrent.futures as cf
+import hashlib
+import os
+import shutil
+import subprocess
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(HERE)
+GEN = os.path.join(HERE, "_gen")
+OBJ = os.path.join(HERE, "_build")
+LIBDIR = os.path.join(PKG, "lib")
+LIB = os.path.join(LIBDIR, "libquax_hip.so")
+ROOT = os.path.dirname(PKG)
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+FLAGS = ["-std=c++17", "-O3", f"--offload-arch={ARCH}", "-fPIC", "-munsafe-fp-atomics",
+         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         "-Wno-unused-variable", "-Wno-unused-but-set-variable",
+         f"-I{os.path.join(ROOT, 'include')}", f"-I{HERE}"]
+
+RUNTIME_SOURCES = ["qx_runtime.hip", "qx_kernels.hip"]
+HEADERS = ["qx_device.h", "qx_runtime.h"]
+
+
+def _hash(paths, extra):
+    h = hashlib.sha256()
+    for p in paths:
+        with open(p, "rb") as fh:
+            h.update(fh.read())
+    h.update(" ".join(extra).encode())
+    return h.hexdigest()[:20]
+
+
+def _compile(src, force):
+    if src.startswith(GEN):
+        # generated class kernels include only qx_device.h
+        deps = [src, os.path.join(HERE, "qx_device.h"), os.path.join(GEN, "qx_classes.inc")]
+    else:
+        deps = [src] + [os.path.join(HERE, h) for h in HEADERS] + [os.path.join(ROOT, "include", "quax_amd.h"),
+                                                                   os.path.join(GEN, "qx_classes.inc")]
+    key = _hash([d for d in deps if os.path.exists(d)], FLAGS)
+    obj = os.path.join(OBJ, os.path.basename(src) + "." + key + ".o")
+    if os.path.exists(obj) and not force:
+        return obj, 0.0, True
+    t0 = time.time()
+    cmd = [HIPCC] + FLAGS + ["-c", src, "-o", obj + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr[-4000:]}")
+    os.replace(obj + ".tmp", obj)
+    return obj, time.time() - t0, False
+
+
+def build(force=False, jobs=None, lmax=2, fp32=False, verbose=True):
+    os.makedirs(OBJ, exist_ok=True)
+    os.makedirs(LIBDIR, exist_ok=True)
+    gen_args = [sys.executable, os.path.join(HERE, "gen_eri.py"), GEN, str(lmax)] + (["--fp32"] if fp32 else [])
+    if os.path.isdir(GEN):
+        shutil.rmtree(GEN)
+    subprocess.run(gen_args, check=True, capture_output=True)
+    gen_srcs = sorted(os.path.join(GEN, f) for f in os.listdir(GEN) if f.endswith(".hip"))
+    # biggest first so the long poles start early
+    gen_srcs.sort(key=lambda p: -os.path.getsize(p))
+    srcs = [os.path.join(HERE, s) for s in RUNTIME_SOURCES] + gen_srcs
+    jobs = jobs or min(16, os.cpu_count() or 4)
+    t0 = time.time()
+    objs = []
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        futs = {ex.submit(_compile, s, force): s for s in srcs}
+        for f in cf.as_completed(futs):
+            obj, dt, cached = f.result()
+            objs.append(obj)
+            if verbose and not cached:
+                print(f"  compiled {os.path.basename(futs[f])} in {dt:.1f} s", flush=True)
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", LIB + ".tmp"] + sorted(objs)
+    subprocess.run(cmd, check=True)
+    os.replace(LIB + ".tmp", LIB)
+    if verbose:
+        print(f"built {LIB} ({len(objs)} objects, {time.time() - t0:.1f} s)", flush=True)
+    return LIB
+
+
+if __name__ == "__main__":
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--force", action="store_true")
+    ap.add_argument("--jobs", type=int, default=None)
+    ap.add_argument("--lmax", type=int, default=2)
+    ap.add_argument("--fp32", action="store_true")
+    a = ap.parse_args()
+    build(force=a.force, jobs=a.jobs, lmax=a.lmax, fp32=a.fp32)

@@ -1,0 +1,403 @@
+"""Generates the per-angular-momentum-class ERI and ERI-derivative kernels.
+
+One ERI kernel per canonical class (la lb | lc ld) with la >= lb, lc >= ld and
+bra pair class >= ket pair class.  Each thread evaluates one primitive-pair
+quartet (contracted shells are handled by summing primitive-pair quartets in
+the epilogue), as straight-line code:
+
+  1. setup (P, Q, W, rho) and the Boys function F_0..F_L,
+  2. Obara-Saika vertical recurrence [e0|f0]^(m), emitted as SSA scalars in
+     dependency order (bra side first, then transferred to the ket),
+  3. Head-Gordon-Pople horizontal recurrence to (ab|cd), choosing the cheaper
+     of bra-first / ket-first per class,
+  4. an epilogue selected at run time (store, full-tensor scatter, Schwarz
+     diagonal, direct J/K).
+
+Derivative kernels build d/dX_i (ab|cd) from d/dA_i phi_a = 2 alpha
+phi_{a+1_i} - a_i phi_{a-1_i}.  Light classes use one kernel that forms
+d/dA, d/dB, d/dC and infers d/dD by translational invariance; heavy classes
+are split into parts (one centre, or one centre and one Cartesian direction,
+per kernel) so that no single straight-line body becomes too large to
+register-allocate.  Epilogues: full-tensor scatter or the adjoint gradient.
+
+Usage: python gen_eri.py OUTDIR [LMAX]
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+AX = "xyz"
+# largest SSA node count allowed in one derivative body before it is split
+SPLIT_NODES = 7000
+
+
+def comps(l):
+    out = []
+    for i in range(l + 1):
+        lx = l - i
+        for j in range(i + 1):
+            out.append((lx, i - j, j))
+    return out
+
+
+def ncart(l):
+    return (l + 1) * (l + 2) // 2
+
+
+def tot(v):
+    return v[0] + v[1] + v[2]
+
+
+def inc(v, i, d=1):
+    w = list(v)
+    w[i] += d
+    return tuple(w)
+
+
+def first_axis(v):
+    for i in range(3):
+        if v[i] > 0:
+            return i
+    return -1
+
+
+Z3 = (0, 0, 0)
+
+
+class VRR:
+    """Obara-Saika vertical recurrence DAG for one primitive quartet."""
+
+    def __init__(self):
+        self.memo = {}
+        self.lines = []
+        self.nflop = 0
+
+    def get(self, e, f, m):
+        key = (e, f, m)
+        if key in self.memo:
+            return self.memo[key]
+        if tot(f) > 0:
+            i = first_axis(f)
+            f1 = inc(f, i, -1)
+            terms = [f"QC{AX[i]} * {self.get(e, f1, m)}", f"WQ{AX[i]} * {self.get(e, f1, m + 1)}"]
+            self.nflop += 3
+            if f1[i] > 0:
+                f2 = inc(f1, i, -1)
+                terms.append(f"R({f1[i]}) * oo2q * ({self.get(e, f2, m)} - rq * {self.get(e, f2, m + 1)})")
+                self.nflop += 5
+            if e[i] > 0:
+                terms.append(f"R({e[i]}) * oo2pq * {self.get(inc(e, i, -1), f1, m + 1)}")
+                self.nflop += 3
+        elif tot(e) > 0:
+            i = first_axis(e)
+            e1 = inc(e, i, -1)
+            terms = [f"PA{AX[i]} * {self.get(e1, Z3, m)}", f"WP{AX[i]} * {self.get(e1, Z3, m + 1)}"]
+            self.nflop += 3
+            if e1[i] > 0:
+                e2 = inc(e1, i, -1)
+                terms.append(f"R({e1[i]}) * oo2p * ({self.get(e2, Z3, m)} - rp * {self.get(e2, Z3, m + 1)})")
+                self.nflop += 5
+        else:
+            name = f"b{m}"
+            self.memo[key] = name
+            return name
+        name = f"v{len(self.memo)}"
+        self.memo[key] = name
+        self.lines.append(f"const R {name} = {' + '.join(terms)};")
+        return name
+
+
+class HRR:
+    """Head-Gordon-Pople horizontal recurrence DAG.  order 'b' reduces b
+    before d (so the ket HRR executes first), order 'd' the reverse."""
+
+    def __init__(self, order):
+        self.order = order
+        self.memo = {}
+        self.lines = []
+        self.need = set()
+        self.nflop = 0
+
+    def get(self, a, b, c, d):
+        key = (a, b, c, d)
+        if key in self.memo:
+            return self.memo[key]
+        red_b = tot(b) > 0 and (self.order == "b" or tot(d) == 0)
+        if red_b:
+            i = first_axis(b)
+            b1 = inc(b, i, -1)
+            expr = f"{self.get(inc(a, i), b1, c, d)} + AB{AX[i]} * {self.get(a, b1, c, d)}"
+            self.nflop += 2
+        elif tot(d) > 0:
+            i = first_axis(d)
+            d1 = inc(d, i, -1)
+            expr = f"{self.get(a, b, inc(c, i), d1)} + CD{AX[i]} * {self.get(a, b, c, d1)}"
+            self.nflop += 2
+        else:
+            self.need.add((a, c))
+            name = f"s_{a[0]}{a[1]}{a[2]}_{c[0]}{c[1]}{c[2]}"
+            self.memo[key] = name
+            return name
+        name = f"h{len(self.memo)}"
+        self.memo[key] = name
+        self.lines.append(f"const R {name} = {expr};")
+        return name
+
+
+def build_dag(targets):
+    best = None
+    for order in "bd":
+        h = HRR(order)
+        outs = [h.get(*t) for t in targets]
+        v = VRR()
+        vn = {ac: v.get(ac[0], ac[1], 0) for ac in sorted(h.need)}
+        cost = len(h.lines) + len(v.lines)
+        if best is None or cost < best[0]:
+            best = (cost, h, v, vn, outs)
+    return best[1:]
+
+
+CENTRES = "ABCD"
+
+
+def deriv_targets(l, part):
+    """Targets of one derivative part, in emission order: for every
+    (a,b,c,d), for every (centre, x) of the part: raised, then lowered."""
+    la, lb, lc, ld = l
+    T = []
+    for a in comps(la):
+        for b in comps(lb):
+            for c in comps(lc):
+                for d in comps(ld):
+                    for (cen, x) in part:
+                        q = [a, b, c, d]
+                        k = CENTRES.index(cen)
+                        up = list(q)
+                        up[k] = inc(q[k], x)
+                        T.append(tuple(up))
+                        if q[k][x] > 0:
+                            dn = list(q)
+                            dn[k] = inc(q[k], x, -1)
+                            T.append(tuple(dn))
+    return T
+
+
+def deriv_parts(l):
+    """Choose how to split the derivative of class l into kernels."""
+    full = [(c, x) for c in "ABC" for x in range(3)]
+    h, v, _, _ = build_dag(deriv_targets(l, full))
+    if len(h.lines) + len(v.lines) <= SPLIT_NODES:
+        return [("ABC", "xyz", True, full)]
+    parts = []
+    for c in CENTRES:
+        pc = [(c, x) for x in range(3)]
+        h, v, _, _ = build_dag(deriv_targets(l, pc))
+        if len(h.lines) + len(v.lines) <= SPLIT_NODES:
+            parts.append((c, "xyz", False, pc))
+        else:
+            for x in range(3):
+                parts.append((c, AX[x], False, [(c, x)]))
+    return parts
+
+
+def emit_setup(L, Lt, deriv_w):
+    L.append("  const double p = A_.pp.p[u1], Kab = A_.pp.K[u1];")
+    L.append("  const double Px = A_.pp.Px[u1], Py = A_.pp.Py[u1], Pz = A_.pp.Pz[u1];")
+    L.append("  const double q = A_.pp.p[u2], Kcd = A_.pp.K[u2];")
+    L.append("  const double Qx = A_.pp.Px[u2], Qy = A_.pp.Py[u2], Qz = A_.pp.Pz[u2];")
+    L.append("  const double Ax = A_.pr.Ax[P1], Ay = A_.pr.Ay[P1], Az = A_.pr.Az[P1];")
+    L.append("  const double Cx = A_.pr.Ax[P2], Cy = A_.pr.Ay[P2], Cz = A_.pr.Az[P2];")
+    L.append("  const R ABx = (R)A_.pr.ABx[P1], ABy = (R)A_.pr.ABy[P1], ABz = (R)A_.pr.ABz[P1];")
+    L.append("  const R CDx = (R)A_.pr.ABx[P2], CDy = (R)A_.pr.ABy[P2], CDz = (R)A_.pr.ABz[P2];")
+    L.append("  const double pq = p + q, opq = 1.0 / pq;")
+    L.append("  const double Wx = (p * Px + q * Qx) * opq, Wy = (p * Py + q * Qy) * opq, Wz = (p * Pz + q * Qz) * opq;")
+    L.append("  const double PQx = Px - Qx, PQy = Py - Qy, PQz = Pz - Qz;")
+    L.append("  const double rho = p * q * opq;")
+    L.append("  const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);")
+    L.append(f"  double F[{Lt + 1}];")
+    L.append(f"  qx_boys<{Lt}>(T, A_.boys, F);")
+    L.append("  const double pref = QX_TWO_PI_5_2 * Kab * Kcd / (p * q * sqrt(pq));")
+    for m in range(Lt + 1):
+        L.append(f"  const R b{m} = (R)(pref * F[{m}]);")
+
+
+def emit_vrr_consts(L, vrr_lines):
+    used = "\n".join(vrr_lines)
+    for ax in AX:
+        if f"PA{ax}" in used:
+            L.append(f"  const R PA{ax} = (R)(P{ax} - A{ax});")
+        if f"WP{ax}" in used:
+            L.append(f"  const R WP{ax} = (R)(W{ax} - P{ax});")
+        if f"QC{ax}" in used:
+            L.append(f"  const R QC{ax} = (R)(Q{ax} - C{ax});")
+        if f"WQ{ax}" in used:
+            L.append(f"  const R WQ{ax} = (R)(W{ax} - Q{ax});")
+    if "oo2p " in used:
+        L.append("  const R oo2p = (R)(0.5 / p);")
+    if "oo2q " in used:
+        L.append("  const R oo2q = (R)(0.5 / q);")
+    if "oo2pq " in used:
+        L.append("  const R oo2pq = (R)(0.5 * opq);")
+    if " rp " in used:
+        L.append("  const R rp = (R)(q * opq);")
+    if " rq " in used:
+        L.append("  const R rq = (R)(p * opq);")
+
+
+def emit_eri_core(l):
+    la, lb, lc, ld = l
+    N = ncart(la) * ncart(lb) * ncart(lc) * ncart(ld)
+    T = [(a, b, c, d) for a in comps(la) for b in comps(lb) for c in comps(lc) for d in comps(ld)]
+    h, v, vn, outs = build_dag(T)
+    Lt = sum(l)
+    nm = "".join(map(str, l))
+    L = [f"template <typename R>\n__device__ __forceinline__ void qx_core_e{nm}(const QxArgs& A_, int P1, int P2, "
+         f"int u1, int u2, R (&o)[{N}]) {{"]
+    emit_setup(L, Lt, False)
+    emit_vrr_consts(L, v.lines)
+    L += ["  " + x for x in v.lines]
+    for (a, c), name in sorted(vn.items()):
+        L.append(f"  const R s_{a[0]}{a[1]}{a[2]}_{c[0]}{c[1]}{c[2]} = {name};")
+    L += ["  " + x for x in h.lines]
+    for k, o in enumerate(outs):
+        L.append(f"  o[{k}] = {o};")
+    L.append("}")
+    stats = dict(vrr=len(v.lines), vrr_flops=v.nflop, hrr=len(h.lines), hrr_flops=h.nflop, L=Lt, N=N,
+                 flops=core_flops(Lt, v.nflop, h.nflop))
+    return "\n".join(L), stats
+
+
+def emit_deriv_core(l, pi, part):
+    la, lb, lc, ld = l
+    N = ncart(la) * ncart(lb) * ncart(lc) * ncart(ld)
+    cen, xs, dinv, combos = part
+    T = deriv_targets(l, combos)
+    h, v, vn, outs = build_dag(T)
+    Lt = sum(l) + 1
+    nm = "".join(map(str, l))
+    NC = len(combos)
+    L = [f"template <typename R>\n__device__ __forceinline__ void qx_core_d{nm}_{pi}(const QxArgs& A_, int P1, int P2, "
+         f"int u1, int u2, R (&dv)[{NC * N}]) {{"]
+    emit_setup(L, Lt, True)
+    emit_vrr_consts(L, v.lines)
+    L += ["  " + x for x in v.lines]
+    for (a, c), name in sorted(vn.items()):
+        L.append(f"  const R s_{a[0]}{a[1]}{a[2]}_{c[0]}{c[1]}{c[2]} = {name};")
+    L += ["  " + x for x in h.lines]
+    wexpr = {"A": "(R)A_.pp.a2[u1]", "B": "(R)A_.pp.b2[u1]", "C": "(R)A_.pp.a2[u2]", "D": "(R)A_.pp.b2[u2]"}
+    for c in sorted(set(cc for cc, _ in combos)):
+        L.append(f"  const R w{c} = {wexpr[c]};")
+    it = iter(outs)
+    e = 0
+    for a in comps(la):
+        for b in comps(lb):
+            for c in comps(lc):
+                for d in comps(ld):
+                    q = [a, b, c, d]
+                    for ci, (cn, x) in enumerate(combos):
+                        k = CENTRES.index(cn)
+                        up = next(it)
+                        n = q[k][x]
+                        dn = next(it) if n > 0 else None
+                        expr = f"w{cn} * {up}" + (f" - R({n}) * {dn}" if dn else "")
+                        L.append(f"  dv[{ci * N + e}] = {expr};")
+                    e += 1
+    L.append("}")
+    cmask = sum(1 << CENTRES.index(c) for c in set(cc for cc, _ in combos))
+    xmask = sum(1 << x for x in set(x for _, x in combos))
+    stats = dict(part=pi, centres=cen, xs=xs, dinv=dinv, vrr=len(v.lines), vrr_flops=v.nflop,
+                 hrr=len(h.lines), hrr_flops=h.nflop, L=Lt, N=N, ncomb=NC,
+                 flops=core_flops(Lt, v.nflop, h.nflop) + 3 * NC * N)
+    return "\n".join(L), stats, (cmask, xmask, dinv, NC)
+
+
+def core_flops(L, vrr_flops, hrr_flops):
+    """Algorithmic FP operations of one primitive-quartet body (mul and add
+    count 1, fused multiply-add 2): 30 for the pair/quartet setup, 12 + 4L
+    for the Boys function (Taylor step plus downward recursion), then the
+    recurrences as emitted."""
+    return 30 + 12 + 4 * L + vrr_flops + hrr_flops
+
+
+def canonical_classes(lmax):
+    pairs = [(a, b) for a in range(lmax + 1) for b in range(a + 1)]
+    out = []
+    for i, (la, lb) in enumerate(pairs):
+        for j, (lc, ld) in enumerate(pairs):
+            if j <= i:
+                out.append((la, lb, lc, ld))
+    return pairs, out
+
+
+def emit_class_files(cls, outdir, fp32):
+    """Writes cls_<l>_e.hip and one cls_<l>_d<k>.hip per derivative part."""
+    la, lb, lc, ld = cls
+    nm = "".join(map(str, cls))
+    dims = f"{ncart(la)}, {ncart(lb)}, {ncart(lc)}, {ncart(ld)}"
+    N = ncart(la) * ncart(lb) * ncart(lc) * ncart(ld)
+    files, stats = [], []
+    src, st = emit_eri_core(cls)
+    body = ['#include "../qx_device.h"', "", src, "",
+            f"QX_DEFINE_ERI_KERNEL(e{nm}, {N}, {dims}, {1 if fp32 else 0})", ""]
+    path = os.path.join(outdir, f"cls_{nm}_e.hip")
+    with open(path, "w") as fh:
+        fh.write("\n".join(body))
+    files.append(path)
+    stats.append((cls, "eri", st))
+    parts = deriv_parts(cls)
+    for pi, part in enumerate(parts):
+        src, st, (cmask, xmask, dinv, NC) = emit_deriv_core(cls, pi, part)
+        body = ['#include "../qx_device.h"', "", src, "",
+                f"QX_DEFINE_DERI_KERNEL(d{nm}_{pi}, {N}, {dims}, {cmask}, {xmask}, {1 if dinv else 0}, {NC}, "
+                f"{1 if fp32 else 0})", ""]
+        path = os.path.join(outdir, f"cls_{nm}_d{pi}.hip")
+        with open(path, "w") as fh:
+            fh.write("\n".join(body))
+        files.append(path)
+        stats.append((cls, f"deriv{pi}", st))
+    return files, stats, len(parts)
+
+
+def main():
+    outdir = sys.argv[1]
+    lmax = int(sys.argv[2]) if len(sys.argv) > 2 else 2
+    fp32 = "--fp32" in sys.argv
+    os.makedirs(outdir, exist_ok=True)
+    _, classes = canonical_classes(lmax)
+    files, allstats, nparts, flops = [], [], {}, {}
+    for cls in classes:
+        f, s, np_ = emit_class_files(cls, outdir, fp32)
+        files += f
+        allstats += s
+        nparts[cls] = np_
+        flops[cls] = (s[0][2]["flops"], [st["flops"] for _, _, st in s[1:]], [st["ncomb"] for _, _, st in s[1:]])
+    D = ["// generated: kernel launch tables indexed by canonical class id", "#pragma once",
+         f"#define QX_GEN_LMAX {lmax}", f"#define QX_NUM_CLASSES {len(classes)}",
+         f"#define QX_MAX_DPARTS {max(nparts.values())}", ""]
+    for cls in classes:
+        nm = "".join(map(str, cls))
+        D.append(f'extern "C" int qx_launch_e{nm}(const QxArgs*, const QxLaunch*);')
+        for pi in range(nparts[cls]):
+            D.append(f'extern "C" int qx_launch_d{nm}_{pi}(const QxArgs*, const QxLaunch*);')
+    D.append("")
+    D.append("static const QxClassEntry qx_class_table[QX_NUM_CLASSES] = {")
+    for cls in classes:
+        nm = "".join(map(str, cls))
+        dl = ", ".join(f"qx_launch_d{nm}_{pi}" for pi in range(nparts[cls]))
+        ef, pf, pc = flops[cls]
+        pfs = ", ".join(f"{x:.1f}" for x in pf)
+        pcs = ", ".join(str(x) for x in pc)
+        D.append(f"  {{{cls[0]}, {cls[1]}, {cls[2]}, {cls[3]}, qx_launch_e{nm}, {nparts[cls]}, {{{dl}}}, "
+                 f"{ef:.1f}, {{{pfs}}}, {{{pcs}}}}},")
+    D.append("};")
+    with open(os.path.join(outdir, "qx_classes.inc"), "w") as fh:
+        fh.write("\n".join(D) + "\n")
+    with open(os.path.join(outdir, "stats.txt"), "w") as fh:
+        for cls, kind, st in allstats:
+            fh.write(f"{cls} {kind} {st}\n")
+    print("\n".join(files))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,446 @@
+// Device-side definitions shared by the generated per-class ERI kernels:
+// argument structs, the Boys function, quartet lookup and the epilogues.
+//
+// Work decomposition.  Screened shell pairs are sorted by class and, within a
+// class, by descending Schwarz bound.  Each contracted pair expands into its
+// primitive pairs ("pseudo pairs", stored contiguously per pair).  One thread
+// evaluates one pseudo-pair quartet; a contracted quartet is the sum of its
+// pseudo-pair quartets, so every epilogue that is linear in the integrals
+// (J/K, gradient, tensor scatter with atomics) contracts by accumulation.
+// For a bra pseudo pair i the admissible ket pseudo pairs form a prefix of the
+// ket list (Schwarz-sorted, canonical j <= i within one class), so a work list
+// is a running-count array over bra pseudo pairs and thread t finds its bra by
+// binary search.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define QX_BLOCK 128
+#define QX_TWO_PI_5_2 34.986836655249725  // 2 pi^(5/2)
+#define QX_PI 3.14159265358979323846
+
+// Boys table: F_m(T_g) for T_g = g / QX_BOYS_INV_H, g = 0..QX_BOYS_NPTS-1,
+// m = 0..QX_BOYS_NM-1.  A seven-term Taylor expansion about the nearest grid
+// point (|T - T_g| <= 1/64) gives F_M; lower orders follow by downward
+// recursion.  Above QX_BOYS_TMAX: asymptotic F_0 and upward recursion.
+#define QX_BOYS_TMAX 36.0
+#define QX_BOYS_INV_H 32.0
+#define QX_BOYS_NM 24
+#define QX_BOYS_NPTS (36 * 32 + 2)
+#define QX_BOYS_MMAX (QX_BOYS_NM - 7)
+
+// largest Cartesian shell-pair block (f x f); sizes the Schwarz diagonal rows
+#define QX_PAIR_STRIDE 100
+
+enum QxEpi {
+  QX_EPI_STORE = 0,    // out[store_base + comp * nquart + t] (single-primitive bases)
+  QX_EPI_FULL = 1,     // scatter all distinct permutations into the nbf^4 tensor
+  QX_EPI_SCHWARZ = 2,  // diagonal (f1 f2|f1 f2) of pair P into out[P * stride + f1*nb + f2]
+  QX_EPI_JK = 3,       // direct J', K' accumulation with the density
+  QX_EPI_DFULL = 4,    // derivative tensor scatter (all 3N slices, or one)
+  QX_EPI_GRAD = 5      // adjoint gradient: sum dERI * density pair products
+};
+
+struct QxPairsDev {        // screened shell pairs in sorted order (SoA)
+  const double* Ax; const double* Ay; const double* Az;      // centre of shell a
+  const double* ABx; const double* ABy; const double* ABz;   // A - B
+  const int* sa; const int* sb;                              // shell indices
+  const int* bfa; const int* bfb;                            // first Cartesian function
+  const int* atom_a; const int* atom_b;
+  const int* pp_first; const int* pp_n;                      // pseudo-pair range
+};
+
+struct QxPrimPairsDev {    // pseudo pairs (one primitive pair each), SoA
+  const int* pair;                                           // owning pair
+  const double* p;                                           // alpha + beta
+  const double* Px; const double* Py; const double* Pz;      // product centre
+  const double* K;      // c_a c_b exp(-alpha beta |AB|^2 / p)
+  const double* a2;     // 2 alpha
+  const double* b2;     // 2 beta
+};
+
+struct QxWorkDev {         // one (bra class, ket class) quartet list
+  int bra_off, nbra;         // bra pseudo-pair range
+  int ket_off, nket;         // ket pseudo-pair range
+  const long long* prefix;   // nbra+1 running quartet counts
+  long long nquart;
+  long long store_base;      // element offset of this list in the store buffer
+  int diag;                  // kets are the pseudo pairs of the bra's own pair
+};
+
+struct QxEpiDev {
+  int epi;
+  int atomic;                // STORE/FULL: accumulate with atomics
+  int nbf, natom;
+  int coord;                 // DFULL: -1 = all 3N slices, else one coordinate
+  double* out;
+  const double* D;           // density, Cartesian nbf x nbf (JK, GRAD)
+  double* J; double* K;      // unsymmetrised J', K' (JK)
+  double* grad;              // 3N accumulators (GRAD)
+};
+
+struct QxArgs {
+  QxPairsDev pr;
+  QxPrimPairsDev pp;
+  QxWorkDev w;
+  QxEpiDev e;
+  const double* boys;
+};
+
+struct QxLaunch {
+  int fp32;                  // recursion in FP32 (JK / GRAD epilogues only)
+  hipStream_t stream;
+};
+
+template <int M>
+__device__ __forceinline__ void qx_boys(double T, const double* __restrict__ tab, double (&F)[M + 1]) {
+  static_assert(M <= QX_BOYS_MMAX, "Boys order beyond table");
+  if (T < QX_BOYS_TMAX) {
+    const int g = (int)(T * QX_BOYS_INV_H + 0.5);
+    const double d = g * (1.0 / QX_BOYS_INV_H) - T;
+    const double* r = tab + g * QX_BOYS_NM + M;
+    double f = r[6] * (1.0 / 720.0);
+    f = f * d + r[5] * (1.0 / 120.0);
+    f = f * d + r[4] * (1.0 / 24.0);
+    f = f * d + r[3] * (1.0 / 6.0);
+    f = f * d + r[2] * 0.5;
+    f = f * d + r[1];
+    f = f * d + r[0];
+    F[M] = f;
+    if constexpr (M > 0) {
+      const double e = exp(-T);
+      const double t2 = 2.0 * T;
+#pragma unroll
+      for (int m = M - 1; m >= 0; --m) F[m] = (t2 * F[m + 1] + e) * (1.0 / (2 * m + 1));
+    }
+  } else {
+    F[0] = 0.5 * sqrt(QX_PI / T);
+    if constexpr (M > 0) {
+      const double e = exp(-T);
+      const double o2t = 0.5 / T;
+#pragma unroll
+      for (int m = 0; m < M; ++m) F[m + 1] = ((2 * m + 1) * F[m] - e) * o2t;
+    }
+  }
+}
+
+__device__ __forceinline__ void qx_locate(const QxArgs& A_, long long t, int& u1, int& u2, int& P1, int& P2) {
+  const long long* pre = A_.w.prefix;
+  int lo = 0, hi = A_.w.nbra;  // pre[lo] <= t < pre[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (pre[mid] <= t) lo = mid; else hi = mid;
+  }
+  const int local = (int)(t - pre[lo]);
+  u1 = A_.w.bra_off + lo;
+  P1 = A_.pp.pair[u1];
+  u2 = A_.w.diag ? A_.pr.pp_first[P1] + local : A_.w.ket_off + local;
+  P2 = A_.pp.pair[u2];
+}
+
+__device__ __forceinline__ void qx_atomic_add(double* p, double v) { unsafeAtomicAdd(p, v); }
+
+__device__ __forceinline__ void qx_put(double* p, double v, bool atomic) {
+  if (atomic) qx_atomic_add(p, v); else *p = v;
+}
+
+// Degeneracy weight of a canonical quartet: 1/2 for each of a==b, c==d and
+// (ab)==(cd), so that summing all 8 index permutations with this weight
+// counts every distinct permutation once.
+__device__ __forceinline__ double qx_degeneracy(const QxArgs& A_, int P1, int P2) {
+  double f = 1.0;
+  if (A_.pr.sa[P1] == A_.pr.sb[P1]) f *= 0.5;
+  if (A_.pr.sa[P2] == A_.pr.sb[P2]) f *= 0.5;
+  if (P1 == P2) f *= 0.5;
+  return f;
+}
+
+// Writes v to every distinct index permutation of (I J | K L).  Permutations
+// that coincide with another element of the same shell block (a==b, c==d,
+// (ab)==(cd)) are skipped: that element writes them itself.
+__device__ __forceinline__ void qx_scatter8(double* out, size_t n, size_t I, size_t J, size_t K, size_t L, double v,
+                                            bool sab, bool scd, bool spair, bool atomic) {
+  qx_put(&out[((I * n + J) * n + K) * n + L], v, atomic);
+  if (!sab) qx_put(&out[((J * n + I) * n + K) * n + L], v, atomic);
+  if (!scd) qx_put(&out[((I * n + J) * n + L) * n + K], v, atomic);
+  if (!sab && !scd) qx_put(&out[((J * n + I) * n + L) * n + K], v, atomic);
+  if (!spair) {
+    qx_put(&out[((K * n + L) * n + I) * n + J], v, atomic);
+    if (!scd) qx_put(&out[((L * n + K) * n + I) * n + J], v, atomic);
+    if (!sab) qx_put(&out[((K * n + L) * n + J) * n + I], v, atomic);
+    if (!sab && !scd) qx_put(&out[((L * n + K) * n + J) * n + I], v, atomic);
+  }
+}
+
+// ---- ERI epilogues ----------------------------------------------------
+template <int NA, int NB, int NC, int ND, typename R>
+__device__ __forceinline__ void qx_epi_jk(const QxArgs& A_, int P1, int P2, const R (&o)[NA * NB * NC * ND]) {
+  // J'_ab += 4f (ab|cd) D_cd, J'_cd += 4f (ab|cd) D_ab,
+  // K'_ac += 2f (ab|cd) D_bd, K'_ad += 2f D_bc, K'_bc += 2f D_ad, K'_bd += 2f D_ac
+  const int n = A_.e.nbf;
+  const int ia = A_.pr.bfa[P1], ib = A_.pr.bfb[P1], ic = A_.pr.bfa[P2], id = A_.pr.bfb[P2];
+  const double* D = A_.e.D;
+  double Dab[NA * NB], Dcd[NC * ND], Dac[NA * NC], Dad[NA * ND], Dbc[NB * NC], Dbd[NB * ND];
+#pragma unroll
+  for (int i = 0; i < NA; ++i) {
+#pragma unroll
+    for (int j = 0; j < NB; ++j) Dab[i * NB + j] = D[(ia + i) * n + ib + j];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) Dac[i * NC + k] = D[(ia + i) * n + ic + k];
+#pragma unroll
+    for (int l = 0; l < ND; ++l) Dad[i * ND + l] = D[(ia + i) * n + id + l];
+  }
+#pragma unroll
+  for (int j = 0; j < NB; ++j) {
+#pragma unroll
+    for (int k = 0; k < NC; ++k) Dbc[j * NC + k] = D[(ib + j) * n + ic + k];
+#pragma unroll
+    for (int l = 0; l < ND; ++l) Dbd[j * ND + l] = D[(ib + j) * n + id + l];
+  }
+#pragma unroll
+  for (int k = 0; k < NC; ++k)
+#pragma unroll
+    for (int l = 0; l < ND; ++l) Dcd[k * ND + l] = D[(ic + k) * n + id + l];
+  double jab[NA * NB] = {}, jcd[NC * ND] = {}, kac[NA * NC] = {}, kad[NA * ND] = {}, kbc[NB * NC] = {},
+         kbd[NB * ND] = {};
+#pragma unroll
+  for (int i = 0; i < NA; ++i)
+#pragma unroll
+    for (int j = 0; j < NB; ++j)
+#pragma unroll
+      for (int k = 0; k < NC; ++k)
+#pragma unroll
+        for (int l = 0; l < ND; ++l) {
+          const double v = (double)o[((i * NB + j) * NC + k) * ND + l];
+          jab[i * NB + j] += v * Dcd[k * ND + l];
+          jcd[k * ND + l] += v * Dab[i * NB + j];
+          kac[i * NC + k] += v * Dbd[j * ND + l];
+          kad[i * ND + l] += v * Dbc[j * NC + k];
+          kbc[j * NC + k] += v * Dad[i * ND + l];
+          kbd[j * ND + l] += v * Dac[i * NC + k];
+        }
+  const double f = qx_degeneracy(A_, P1, P2);
+  const double fj = 4.0 * f, fk = 2.0 * f;
+  double* J = A_.e.J;
+  double* K = A_.e.K;
+#pragma unroll
+  for (int i = 0; i < NA; ++i)
+#pragma unroll
+    for (int j = 0; j < NB; ++j) qx_atomic_add(&J[(ia + i) * n + ib + j], fj * jab[i * NB + j]);
+#pragma unroll
+  for (int k = 0; k < NC; ++k)
+#pragma unroll
+    for (int l = 0; l < ND; ++l) qx_atomic_add(&J[(ic + k) * n + id + l], fj * jcd[k * ND + l]);
+#pragma unroll
+  for (int i = 0; i < NA; ++i) {
+#pragma unroll
+    for (int k = 0; k < NC; ++k) qx_atomic_add(&K[(ia + i) * n + ic + k], fk * kac[i * NC + k]);
+#pragma unroll
+    for (int l = 0; l < ND; ++l) qx_atomic_add(&K[(ia + i) * n + id + l], fk * kad[i * ND + l]);
+  }
+#pragma unroll
+  for (int j = 0; j < NB; ++j) {
+#pragma unroll
+    for (int k = 0; k < NC; ++k) qx_atomic_add(&K[(ib + j) * n + ic + k], fk * kbc[j * NC + k]);
+#pragma unroll
+    for (int l = 0; l < ND; ++l) qx_atomic_add(&K[(ib + j) * n + id + l], fk * kbd[j * ND + l]);
+  }
+}
+
+template <int NA, int NB, int NC, int ND, typename R>
+__device__ __forceinline__ void qx_epi_eri(const QxArgs& A_, long long t, int P1, int P2,
+                                           const R (&o)[NA * NB * NC * ND]) {
+  constexpr int N = NA * NB * NC * ND;
+  const int epi = A_.e.epi;
+  if (epi == QX_EPI_JK) {
+    qx_epi_jk<NA, NB, NC, ND, R>(A_, P1, P2, o);
+  } else if (epi == QX_EPI_STORE) {
+    double* out = A_.e.out + A_.w.store_base + t;
+    const long long nq = A_.w.nquart;
+#pragma unroll
+    for (int k = 0; k < N; ++k) out[k * nq] = (double)o[k];
+  } else if (epi == QX_EPI_SCHWARZ) {
+    double* out = A_.e.out + (size_t)P1 * QX_PAIR_STRIDE;
+#pragma unroll
+    for (int i = 0; i < NA; ++i)
+#pragma unroll
+      for (int j = 0; j < NB; ++j) qx_atomic_add(&out[i * NB + j], (double)o[((i * NB + j) * NC + i) * ND + j]);
+  } else if (epi == QX_EPI_FULL) {
+    const size_t n = A_.e.nbf;
+    const int ia = A_.pr.bfa[P1], ib = A_.pr.bfb[P1], ic = A_.pr.bfa[P2], id = A_.pr.bfb[P2];
+    const bool sab = A_.pr.sa[P1] == A_.pr.sb[P1], scd = A_.pr.sa[P2] == A_.pr.sb[P2], sp = P1 == P2;
+    const bool atomic = A_.e.atomic != 0;
+#pragma unroll
+    for (int i = 0; i < NA; ++i)
+#pragma unroll
+      for (int j = 0; j < NB; ++j)
+#pragma unroll
+        for (int k = 0; k < NC; ++k)
+#pragma unroll
+          for (int l = 0; l < ND; ++l)
+            qx_scatter8(A_.e.out, n, ia + i, ib + j, ic + k, id + l, (double)o[((i * NB + j) * NC + k) * ND + l],
+                        sab, scd, sp, atomic);
+  }
+}
+
+// ---- derivative epilogues --------------------------------------------
+// dv holds NCOMB blocks of N, ordered over centres in CMASK (A=1, B=2, C=4,
+// D=8) then Cartesian directions in XMASK.  With DINV the part covers A, B, C
+// in all directions and d/dD = -(d/dA + d/dB + d/dC).
+__device__ __forceinline__ bool qx_single_atom(const QxArgs& A_, int P1, int P2) {
+  const int a = A_.pr.atom_a[P1];
+  return a == A_.pr.atom_b[P1] && a == A_.pr.atom_a[P2] && a == A_.pr.atom_b[P2];
+}
+
+template <int NA, int NB, int NC, int ND, int CMASK, int XMASK, int DINV, int NCOMB, typename R>
+__device__ __forceinline__ void qx_epi_deri(const QxArgs& A_, int P1, int P2, const R (&dv)[NCOMB * NA * NB * NC * ND]) {
+  constexpr int N = NA * NB * NC * ND;
+  const int at[4] = {A_.pr.atom_a[P1], A_.pr.atom_b[P1], A_.pr.atom_a[P2], A_.pr.atom_b[P2]};
+  const int n = A_.e.nbf;
+  const int ia = A_.pr.bfa[P1], ib = A_.pr.bfb[P1], ic = A_.pr.bfa[P2], id = A_.pr.bfb[P2];
+  if (A_.e.epi == QX_EPI_GRAD) {
+    // g_X = sum_ijkl dX(ij|kl) G_ijkl, G_ijkl = 2 D_ij D_kl - (D_ik D_jl + D_il D_jk) / 2,
+    // weighted by 8f (all permutations of a canonical quartet).
+    const double* D = A_.e.D;
+    double g[NCOMB];
+#pragma unroll
+    for (int c = 0; c < NCOMB; ++c) g[c] = 0.0;
+#pragma unroll
+    for (int i = 0; i < NA; ++i)
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        const double dij = D[(ia + i) * n + ib + j];
+#pragma unroll
+        for (int k = 0; k < NC; ++k) {
+          const double dik = D[(ia + i) * n + ic + k], djk = D[(ib + j) * n + ic + k];
+#pragma unroll
+          for (int l = 0; l < ND; ++l) {
+            const double gm = 2.0 * dij * D[(ic + k) * n + id + l] -
+                              0.5 * (dik * D[(ib + j) * n + id + l] + D[(ia + i) * n + id + l] * djk);
+            const int e = ((i * NB + j) * NC + k) * ND + l;
+#pragma unroll
+            for (int c = 0; c < NCOMB; ++c) g[c] += gm * (double)dv[c * N + e];
+          }
+        }
+      }
+    const double w = 8.0 * qx_degeneracy(A_, P1, P2);
+    double* G = A_.e.grad;
+    int ci = 0;
+    double dsum[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int cen = 0; cen < 4; ++cen) {
+      if (!(CMASK & (1 << cen))) continue;
+#pragma unroll
+      for (int x = 0; x < 3; ++x) {
+        if (!(XMASK & (1 << x))) continue;
+        const double v = w * g[ci++];
+        dsum[x] += v;
+        qx_atomic_add(&G[3 * at[cen] + x], v);
+      }
+    }
+    if (DINV) {
+#pragma unroll
+      for (int x = 0; x < 3; ++x) qx_atomic_add(&G[3 * at[3] + x], -dsum[x]);
+    }
+  } else {  // QX_EPI_DFULL
+    const size_t nn = n;
+    const size_t n4 = nn * nn * nn * nn;
+    const bool sab = A_.pr.sa[P1] == A_.pr.sb[P1], scd = A_.pr.sa[P2] == A_.pr.sb[P2], sp = P1 == P2;
+    const int coord = A_.e.coord;
+#pragma unroll
+    for (int i = 0; i < NA; ++i)
+#pragma unroll
+      for (int j = 0; j < NB; ++j)
+#pragma unroll
+        for (int k = 0; k < NC; ++k)
+#pragma unroll
+          for (int l = 0; l < ND; ++l) {
+            const int e = ((i * NB + j) * NC + k) * ND + l;
+            double dsum[3] = {0.0, 0.0, 0.0};
+            int ci = 0;
+#pragma unroll
+            for (int cen = 0; cen < 4; ++cen) {
+              if (!(CMASK & (1 << cen))) continue;
+#pragma unroll
+              for (int x = 0; x < 3; ++x) {
+                if (!(XMASK & (1 << x))) continue;
+                const double v = (double)dv[(ci++) * N + e];
+                dsum[x] += v;
+                const int slot = 3 * at[cen] + x;
+                if (coord >= 0 && slot != coord) continue;
+                double* base = A_.e.out + (coord >= 0 ? 0 : (size_t)slot * n4);
+                qx_scatter8(base, nn, ia + i, ib + j, ic + k, id + l, v, sab, scd, sp, true);
+              }
+            }
+            if (DINV) {
+#pragma unroll
+              for (int x = 0; x < 3; ++x) {
+                const int slot = 3 * at[3] + x;
+                if (coord >= 0 && slot != coord) continue;
+                double* base = A_.e.out + (coord >= 0 ? 0 : (size_t)slot * n4);
+                qx_scatter8(base, nn, ia + i, ib + j, ic + k, id + l, -dsum[x], sab, scd, sp, true);
+              }
+            }
+          }
+  }
+}
+
+// ---- kernel definitions (instantiated once per generated file) --------
+#define QX_GRID(nq) dim3((unsigned)(((nq) + QX_BLOCK - 1) / QX_BLOCK))
+
+#define QX_DEFINE_ERI_KERNEL(NAME, N, NA, NB, NC, ND, WITH32)                                     \
+  template <typename R>                                                                           \
+  __global__ __launch_bounds__(QX_BLOCK) void qx_k_##NAME(QxArgs A_) {                            \
+    const long long t = (long long)blockIdx.x * QX_BLOCK + threadIdx.x;                          \
+    if (t >= A_.w.nquart) return;                                                                 \
+    int u1, u2, P1, P2;                                                                           \
+    qx_locate(A_, t, u1, u2, P1, P2);                                                             \
+    R o[N];                                                                                       \
+    qx_core_##NAME<R>(A_, P1, P2, u1, u2, o);                                                     \
+    qx_epi_eri<NA, NB, NC, ND, R>(A_, t, P1, P2, o);                                              \
+  }                                                                                               \
+  extern "C" int qx_launch_##NAME(const QxArgs* a, const QxLaunch* L) {                           \
+    if (a->w.nquart <= 0) return 0;                                                               \
+    if (L->fp32) {                                                                                \
+      if (!(WITH32) || a->e.epi != QX_EPI_JK) return -1;                                          \
+      if constexpr (WITH32)                                                                       \
+        hipLaunchKernelGGL(qx_k_##NAME<float>, QX_GRID(a->w.nquart), dim3(QX_BLOCK), 0, L->stream, *a); \
+    } else {                                                                                      \
+      hipLaunchKernelGGL(qx_k_##NAME<double>, QX_GRID(a->w.nquart), dim3(QX_BLOCK), 0, L->stream, *a); \
+    }                                                                                             \
+    return (int)hipGetLastError();                                                                \
+  }
+
+#define QX_DEFINE_DERI_KERNEL(NAME, N, NA, NB, NC, ND, CMASK, XMASK, DINV, NCOMB, WITH32)         \
+  template <typename R>                                                                           \
+  __global__ __launch_bounds__(QX_BLOCK) void qx_k_##NAME(QxArgs A_) {                            \
+    const long long t = (long long)blockIdx.x * QX_BLOCK + threadIdx.x;                          \
+    if (t >= A_.w.nquart) return;                                                                 \
+    int u1, u2, P1, P2;                                                                           \
+    qx_locate(A_, t, u1, u2, P1, P2);                                                             \
+    if (qx_single_atom(A_, P1, P2)) return;                                                       \
+    R dv[NCOMB * N];                                                                              \
+    qx_core_##NAME<R>(A_, P1, P2, u1, u2, dv);                                                    \
+    qx_epi_deri<NA, NB, NC, ND, CMASK, XMASK, DINV, NCOMB, R>(A_, P1, P2, dv);                    \
+  }                                                                                               \
+  extern "C" int qx_launch_##NAME(const QxArgs* a, const QxLaunch* L) {                           \
+    if (a->w.nquart <= 0) return 0;                                                               \
+    if (L->fp32) {                                                                                \
+      if (!(WITH32) || a->e.epi != QX_EPI_GRAD) return -1;                                        \
+      if constexpr (WITH32)                                                                       \
+        hipLaunchKernelGGL(qx_k_##NAME<float>, QX_GRID(a->w.nquart), dim3(QX_BLOCK), 0, L->stream, *a); \
+    } else {                                                                                      \
+      hipLaunchKernelGGL(qx_k_##NAME<double>, QX_GRID(a->w.nquart), dim3(QX_BLOCK), 0, L->stream, *a); \
+    }                                                                                             \
+    return (int)hipGetLastError();                                                                \
+  }
+
+struct QxClassEntry {
+  int la, lb, lc, ld;
+  int (*eri)(const QxArgs*, const QxLaunch*);
+  int nparts;
+  int (*deri[12])(const QxArgs*, const QxLaunch*);
+  double eri_flops;          // algorithmic FP operations of one ERI body (no epilogue)
+  double part_flops[12];     // same per derivative part
+  int part_ncomb[12];        // derivative blocks (centre x direction) per part
+};

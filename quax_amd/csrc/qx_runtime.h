@@ -1,0 +1,101 @@
+// Host-side context of the ERI library and the helper-kernel entry points
+// shared between qx_runtime.hip and qx_kernels.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "qx_device.h"
+
+#define QX_LMAX_SUPPORTED 2
+
+// Per-shell and per-function tables for the one-electron kernels and the
+// spherical transforms.
+struct QxShellsDev {
+  int nshell, natom, ncart, nbf;
+  const int* l; const int* atom; const int* nprim; const int* poff;
+  const int* bfc;                         // first Cartesian function of each shell
+  const int* bfs;                         // first AO (spherical when pure) function
+  const double* exps; const double* coefs;
+  const double* xyz;                      // 3*natom
+  const double* Z;                        // natom
+  const int* f_shell;                     // Cartesian function -> shell
+  const int* f_comp;                      // Cartesian function -> packed (lx,ly,lz)
+  const int* ao_shell;                    // AO function -> shell
+  const double* sph;                      // per-l transform blocks, offsets below
+  int sph_off[QX_LMAX_SUPPORTED + 2];
+  int pure;
+};
+
+struct QxWorkHost {
+  int cls;                 // canonical class index into the kernel table
+  int bra_class, ket_class;
+  QxWorkDev w;             // device view (prefix/kstart point into ctx buffers)
+  long long prefix_off;    // offset of this list's prefix in ctx->d_prefix
+};
+
+struct qx_ctx {
+  int device = 0;
+  int natom = 0;
+  std::vector<double> xyz, Z;
+  int nshell = 0;
+  std::vector<int> l, atom, nprim, poff, bfc, bfs;
+  std::vector<double> exps, coefs;
+  int pure = 0, nbf = 0, ncart = 0;
+  double tol = 1e-12;
+  bool k1 = true;          // every shell has one primitive
+  std::string err;
+
+  // device memory owned by the context
+  std::vector<void*> allocs;
+  double* d_boys = nullptr;
+  QxShellsDev shells{};
+  // screened pairs + pseudo pairs (sorted)
+  QxPairsDev pr{};
+  QxPrimPairsDev pp{};
+  int npairs = 0, npseudo = 0;
+  std::vector<double> Q;                 // Schwarz value per shell pair (ns*ns)
+  std::vector<int> ref_pairs;            // reference-order screened pair list
+  // host copies of the sorted screened pairs
+  std::vector<int> h_pair_sa, h_pair_sb, h_pair_class, h_pp_first, h_pp_n;
+  std::vector<double> h_pair_Q;
+  std::vector<int> class_pair_off, class_pair_cnt;   // per pair class, in sorted pairs
+  std::vector<int> class_pp_off, class_pp_cnt;       // per pair class, in pseudo pairs
+  // quartet lists (one per canonical class)
+  std::vector<QxWorkHost> work;
+  long long* d_prefix = nullptr;
+  long long nquart_total = 0;            // canonical contracted quartets
+  long long npq_total = 0;               // pseudo-pair quartets
+  long long store_total = 0;
+  std::vector<long long> nq_class, ndq_class;   // contracted quartets per class (all / not single-atom)
+
+  // optional per-kernel timing with HIP events on the launch stream
+  bool profiling = false;
+  struct ProfEvent {
+    int slot;
+    hipEvent_t start, stop;
+  };
+  std::vector<ProfEvent> prof_pending;
+  std::vector<hipEvent_t> event_pool;
+  std::vector<double> prof_ms;
+  std::vector<long long> prof_launches, prof_pq;
+};
+
+// profile slots: ERI kernel of class c -> c; derivative part k of class c ->
+// QX_NUM_CLASSES_MAX + c * 12 + k
+#define QX_NUM_CLASSES_MAX 21
+#define QX_PROFILE_SLOTS (QX_NUM_CLASSES_MAX * 13)
+
+// helper kernels (qx_kernels.hip)
+extern "C" {
+int qxk_fill(double* p, long long n, double v, hipStream_t s);
+int qxk_sym_half(const double* in, double* out, int n, hipStream_t s);    // out = (in + in^T)/2
+int qxk_cart_to_ao_mat(const QxShellsDev* sh, const double* in, double* out, hipStream_t s);   // T X T^T
+int qxk_ao_to_cart_mat(const QxShellsDev* sh, const double* in, double* out, hipStream_t s);   // T^T X T
+int qxk_cart_to_ao_axis(const QxShellsDev* sh, const double* in, double* out, long long outer, int axis_len_in,
+                        int axis_len_out, long long inner, hipStream_t s);
+int qxk_oei(const QxShellsDev* sh, double* S, double* T, double* V, hipStream_t s);
+int qxk_oei_deriv(const QxShellsDev* sh, const double* Pd, const double* Wd, double* grad, double* dS, double* dT,
+                  double* dV, hipStream_t s);
+}

@@ -1,0 +1,913 @@
+// Host runtime of the MI355X ERI library: context set-up (normalisation,
+// Boys table, shell pairs, Schwarz screening on the device, quartet lists)
+// and the C-ABI entry points declared in include/quax_amd.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "quax_amd.h"
+#include "qx_runtime.h"
+#include "_gen/qx_classes.inc"
+
+static_assert(QX_GEN_LMAX == QX_LMAX_SUPPORTED, "generated kernels and runtime disagree on lmax");
+static_assert(QX_NUM_CLASSES == QX_NUM_CLASSES_MAX, "profile slot layout");
+
+namespace {
+
+struct QxError {
+  int code;
+  std::string msg;
+};
+
+#define QX_HIP(call)                                                                            \
+  do {                                                                                          \
+    hipError_t e_ = (call);                                                                     \
+    if (e_ != hipSuccess)                                                                       \
+      throw QxError{e_ == hipErrorOutOfMemory ? QX_ENOMEM : QX_EHIP,                            \
+                    std::string(#call) + ": " + hipGetErrorString(e_)};                         \
+  } while (0)
+
+#define QX_REQUIRE(cond, code, msg) \
+  do {                              \
+    if (!(cond)) throw QxError{code, msg}; \
+  } while (0)
+
+int ncart(int l) { return (l + 1) * (l + 2) / 2; }
+int nsph(int l, int pure) { return (pure && l >= 2) ? 2 * l + 1 : ncart(l); }
+int pair_class(int la, int lb) { return la * (la + 1) / 2 + lb; }
+
+template <class T>
+T* dalloc(qx_ctx* c, size_t n) {
+  void* p = nullptr;
+  QX_HIP(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+  c->allocs.push_back(p);
+  return static_cast<T*>(p);
+}
+
+template <class T>
+T* dupload(qx_ctx* c, const std::vector<T>& v) {
+  T* p = dalloc<T>(c, v.size());
+  if (!v.empty()) QX_HIP(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  return p;
+}
+
+// scoped temporary device buffer
+struct DevBuf {
+  void* p = nullptr;
+  explicit DevBuf(size_t bytes) { QX_HIP(hipMalloc(&p, std::max<size_t>(bytes, 8))); }
+  ~DevBuf() { if (p) (void)hipFree(p); }
+  double* d() const { return static_cast<double*>(p); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+};
+
+double dfact(int n) {
+  double r = 1.0;
+  for (int k = n; k > 1; k -= 2) r *= k;
+  return r;
+}
+
+// Libint Shell::renorm convention: x^l normalisation of each primitive,
+// then the contraction rescaled to unit self-overlap.
+void normalize_shell(int l, int np, const double* a, const double* cin, double* cout) {
+  const double pi32 = std::pow(M_PI, 1.5);
+  for (int p = 0; p < np; ++p) {
+    const double two_a = 2.0 * a[p];
+    cout[p] = cin[p] * std::sqrt(std::pow(2.0, l) * std::pow(two_a, l + 1.5) / (pi32 * dfact(2 * l - 1)));
+  }
+  double s = 0.0;
+  for (int p = 0; p < np; ++p)
+    for (int q = 0; q < np; ++q)
+      s += dfact(2 * l - 1) * pi32 * cout[p] * cout[q] / (std::pow(2.0, l) * std::pow(a[p] + a[q], l + 1.5));
+  const double f = 1.0 / std::sqrt(s);
+  for (int p = 0; p < np; ++p) cout[p] *= f;
+}
+
+// Boys table rows F_0..F_{NM-1}(g/INV_H): series at the top order in long
+// double, F_m(T) = exp(-T) sum_k (2T)^k / ((2m+1)(2m+3)...(2m+2k+1)),
+// then downward recursion F_{m-1} = (2T F_m + exp(-T)) / (2m-1).
+std::vector<double> boys_table() {
+  std::vector<double> tab((size_t)QX_BOYS_NPTS * QX_BOYS_NM);
+  for (int g = 0; g < QX_BOYS_NPTS; ++g) {
+    const long double T = g / (long double)QX_BOYS_INV_H;
+    const int M = QX_BOYS_NM - 1;
+    long double term = 1.0L / (2 * M + 1), sum = term;
+    for (int k = 1; k < 4000; ++k) {
+      term *= 2.0L * T / (2 * M + 2 * k + 1);
+      sum += term;
+      if (term < 1e-26L * sum) break;
+    }
+    const long double et = expl(-T);
+    long double f = et * sum;
+    tab[(size_t)g * QX_BOYS_NM + M] = (double)f;
+    for (int m = M; m > 0; --m) {
+      f = (2.0L * T * f + et) / (2 * m - 1);
+      tab[(size_t)g * QX_BOYS_NM + m - 1] = (double)f;
+    }
+  }
+  return tab;
+}
+
+// real solid harmonics for l = 2 in terms of x^l-normalised Cartesians
+// (xx, xy, xz, yy, yz, zz), rows m = -2..2, unit-normalised.
+const double kSph2[5 * 6] = {
+    0, 1.7320508075688772, 0, 0, 0, 0,                   // m=-2: sqrt3 xy
+    0, 0, 0, 0, 1.7320508075688772, 0,                   // m=-1: sqrt3 yz
+    -0.5, 0, 0, -0.5, 0, 1.0,                            // m= 0: z^2 - (x^2+y^2)/2
+    0, 0, 1.7320508075688772, 0, 0, 0,                   // m=+1: sqrt3 xz
+    0.8660254037844386, 0, 0, -0.8660254037844386, 0, 0  // m=+2: sqrt3/2 (x^2-y^2)
+};
+
+struct PairRec {
+  int a, b;        // oriented shells (l[a] >= l[b])
+  int s1, s2;      // reference order (s1 >= s2)
+  int cls;
+  double Q;
+};
+
+void build_pair_arrays(qx_ctx* c, const std::vector<PairRec>& P, QxPairsDev& pr, QxPrimPairsDev& pp, int& npseudo,
+                       std::vector<int>* h_first, std::vector<int>* h_n) {
+  const int np = (int)P.size();
+  std::vector<double> Ax(np), Ay(np), Az(np), ABx(np), ABy(np), ABz(np);
+  std::vector<int> sa(np), sb(np), bfa(np), bfb(np), ata(np), atb(np), first(np), cnt(np);
+  std::vector<int> ppair;
+  std::vector<double> p_, Px, Py, Pz, K, a2, b2;
+  for (int i = 0; i < np; ++i) {
+    const int a = P[i].a, b = P[i].b;
+    const double* A = &c->xyz[3 * c->atom[a]];
+    const double* B = &c->xyz[3 * c->atom[b]];
+    Ax[i] = A[0]; Ay[i] = A[1]; Az[i] = A[2];
+    ABx[i] = A[0] - B[0]; ABy[i] = A[1] - B[1]; ABz[i] = A[2] - B[2];
+    sa[i] = a; sb[i] = b;
+    bfa[i] = c->bfc[a]; bfb[i] = c->bfc[b];
+    ata[i] = c->atom[a]; atb[i] = c->atom[b];
+    first[i] = (int)ppair.size();
+    const double AB2 = ABx[i] * ABx[i] + ABy[i] * ABy[i] + ABz[i] * ABz[i];
+    for (int ia = c->poff[a]; ia < c->poff[a] + c->nprim[a]; ++ia)
+      for (int ib = c->poff[b]; ib < c->poff[b] + c->nprim[b]; ++ib) {
+        const double al = c->exps[ia], be = c->exps[ib], p = al + be;
+        ppair.push_back(i);
+        p_.push_back(p);
+        Px.push_back((al * A[0] + be * B[0]) / p);
+        Py.push_back((al * A[1] + be * B[1]) / p);
+        Pz.push_back((al * A[2] + be * B[2]) / p);
+        K.push_back(c->coefs[ia] * c->coefs[ib] * std::exp(-al * be / p * AB2));
+        a2.push_back(2 * al);
+        b2.push_back(2 * be);
+      }
+    cnt[i] = (int)ppair.size() - first[i];
+  }
+  npseudo = (int)ppair.size();
+  pr.Ax = dupload(c, Ax); pr.Ay = dupload(c, Ay); pr.Az = dupload(c, Az);
+  pr.ABx = dupload(c, ABx); pr.ABy = dupload(c, ABy); pr.ABz = dupload(c, ABz);
+  pr.sa = dupload(c, sa); pr.sb = dupload(c, sb);
+  pr.bfa = dupload(c, bfa); pr.bfb = dupload(c, bfb);
+  pr.atom_a = dupload(c, ata); pr.atom_b = dupload(c, atb);
+  pr.pp_first = dupload(c, first); pr.pp_n = dupload(c, cnt);
+  pp.pair = dupload(c, ppair);
+  pp.p = dupload(c, p_); pp.Px = dupload(c, Px); pp.Py = dupload(c, Py); pp.Pz = dupload(c, Pz);
+  pp.K = dupload(c, K); pp.a2 = dupload(c, a2); pp.b2 = dupload(c, b2);
+  if (h_first) *h_first = first;
+  if (h_n) *h_n = cnt;
+}
+
+hipEvent_t take_event(qx_ctx* c) {
+  if (!c->event_pool.empty()) {
+    hipEvent_t e = c->event_pool.back();
+    c->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  QX_HIP(hipEventCreate(&e));
+  return e;
+}
+
+// Runs one kernel launch, bracketed by HIP events on its stream when
+// profiling is on.
+template <class F>
+int timed_launch(qx_ctx* c, int slot, long long pq, hipStream_t s, F&& launch) {
+  if (!c->profiling || pq <= 0) return launch();
+  hipEvent_t a = take_event(c), b = take_event(c);
+  QX_HIP(hipEventRecord(a, s));
+  const int rc = launch();
+  QX_HIP(hipEventRecord(b, s));
+  c->prof_pending.push_back({slot, a, b});
+  c->prof_launches[slot] += 1;
+  c->prof_pq[slot] += pq;
+  return rc;
+}
+
+int launch_eri(qx_ctx* c, const QxWorkDev& w, int cls, const QxEpiDev& e, int fp32, hipStream_t s,
+               const QxPairsDev& pr, const QxPrimPairsDev& pp) {
+  QxArgs a;
+  a.pr = pr; a.pp = pp; a.w = w; a.e = e; a.boys = c->d_boys;
+  QxLaunch L{fp32, s};
+  return timed_launch(c, cls, w.nquart, s, [&] { return qx_class_table[cls].eri(&a, &L); });
+}
+
+void check_launch(int rc, const char* what) {
+  if (rc == -1) throw QxError{QX_EUNSUPPORTED, std::string(what) + ": kernel variant not built"};
+  if (rc != 0) throw QxError{QX_EHIP, std::string(what) + ": " + hipGetErrorString((hipError_t)rc)};
+}
+
+int class_index(int X, int Y) { return X * (X + 1) / 2 + Y; }
+
+void schwarz_and_lists(qx_ctx* c) {
+  const int ns = c->nshell;
+  // ---- every shell pair, class-sorted, for the Schwarz pass
+  std::vector<PairRec> all;
+  for (int s1 = 0; s1 < ns; ++s1)
+    for (int s2 = 0; s2 <= s1; ++s2) {
+      PairRec r;
+      r.s1 = s1; r.s2 = s2;
+      if (c->l[s1] >= c->l[s2]) { r.a = s1; r.b = s2; } else { r.a = s2; r.b = s1; }
+      r.cls = pair_class(c->l[r.a], c->l[r.b]);
+      r.Q = 0.0;
+      all.push_back(r);
+    }
+  std::stable_sort(all.begin(), all.end(), [](const PairRec& x, const PairRec& y) { return x.cls < y.cls; });
+  QxPairsDev pr_all{};
+  QxPrimPairsDev pp_all{};
+  int np_all = 0;
+  const size_t alloc_mark = c->allocs.size();
+  std::vector<int> first, cnt;
+  build_pair_arrays(c, all, pr_all, pp_all, np_all, &first, &cnt);
+  const int npair_all = (int)all.size();
+  DevBuf diag((size_t)npair_all * QX_PAIR_STRIDE * sizeof(double));
+  check_launch(qxk_fill(diag.d(), (long long)npair_all * QX_PAIR_STRIDE, 0.0, 0), "fill");
+  // diagonal lists per pair class: bra = pseudo pairs of the class, kets =
+  // the pseudo pairs of the same contracted pair
+  const int nclass = pair_class(QX_LMAX_SUPPORTED, QX_LMAX_SUPPORTED) + 1;
+  std::vector<std::vector<long long>> pre(nclass);
+  std::vector<QxWorkDev> wl(nclass);
+  std::vector<DevBuf*> bufs;
+  for (int X = 0; X < nclass; ++X) {
+    int lo = -1, hi = -1;
+    for (int i = 0; i < npair_all; ++i)
+      if (all[i].cls == X) { if (lo < 0) lo = i; hi = i + 1; }
+    if (lo < 0) continue;
+    const int u0 = first[lo], u1 = first[hi - 1] + cnt[hi - 1];
+    std::vector<long long> p(u1 - u0 + 1, 0);
+    int idx = 0;
+    for (int i = lo; i < hi; ++i)
+      for (int k = 0; k < cnt[i]; ++k, ++idx) p[idx + 1] = p[idx] + cnt[i];
+    DevBuf* b = new DevBuf(p.size() * sizeof(long long));
+    bufs.push_back(b);
+    QX_HIP(hipMemcpy(b->p, p.data(), p.size() * sizeof(long long), hipMemcpyHostToDevice));
+    QxWorkDev w{};
+    w.bra_off = u0; w.nbra = u1 - u0; w.ket_off = 0; w.nket = 0;
+    w.prefix = static_cast<const long long*>(b->p);
+    w.nquart = p.back(); w.store_base = 0; w.diag = 1;
+    QxEpiDev e{};
+    e.epi = QX_EPI_SCHWARZ; e.out = diag.d(); e.nbf = c->ncart; e.natom = c->natom; e.coord = -1;
+    int rc = launch_eri(c, w, class_index(X, X), e, 0, 0, pr_all, pp_all);
+    if (rc != 0) {
+      for (auto* q : bufs) delete q;
+      check_launch(rc, "schwarz");
+    }
+  }
+  hipError_t se = hipDeviceSynchronize();
+  for (auto* q : bufs) delete q;
+  QX_HIP(se);
+  std::vector<double> hd((size_t)npair_all * QX_PAIR_STRIDE);
+  QX_HIP(hipMemcpy(hd.data(), diag.p, hd.size() * sizeof(double), hipMemcpyDeviceToHost));
+  c->Q.assign((size_t)ns * ns, 0.0);
+  double maxq = 0.0;
+  for (int i = 0; i < npair_all; ++i) {
+    const int na = ncart(c->l[all[i].a]), nb = ncart(c->l[all[i].b]);
+    double m = 0.0;
+    for (int k = 0; k < na * nb; ++k) m = std::max(m, std::fabs(hd[(size_t)i * QX_PAIR_STRIDE + k]));
+    all[i].Q = m;
+    c->Q[(size_t)all[i].s1 * ns + all[i].s2] = c->Q[(size_t)all[i].s2 * ns + all[i].s1] = m;
+    maxq = std::max(maxq, m);
+  }
+  // release the all-pairs arrays
+  while (c->allocs.size() > alloc_mark) {
+    (void)hipFree(c->allocs.back());
+    c->allocs.pop_back();
+  }
+  // ---- screening (libint_interface.cc:215-225)
+  const double tol2 = c->tol * c->tol;
+  const double cut = tol2 / maxq;
+  c->ref_pairs.clear();
+  for (int s1 = 0; s1 < ns; ++s1)
+    for (int s2 = 0; s2 <= s1; ++s2)
+      if (c->Q[(size_t)s1 * ns + s2] >= cut) { c->ref_pairs.push_back(s1); c->ref_pairs.push_back(s2); }
+  std::vector<PairRec> scr;
+  for (auto& r : all)
+    if (r.Q >= cut) scr.push_back(r);
+  std::sort(scr.begin(), scr.end(), [](const PairRec& x, const PairRec& y) {
+    if (x.cls != y.cls) return x.cls < y.cls;
+    if (x.Q != y.Q) return x.Q > y.Q;
+    if (x.s1 != y.s1) return x.s1 < y.s1;
+    return x.s2 < y.s2;
+  });
+  build_pair_arrays(c, scr, c->pr, c->pp, c->npseudo, &c->h_pp_first, &c->h_pp_n);
+  c->npairs = (int)scr.size();
+  c->h_pair_sa.resize(c->npairs); c->h_pair_sb.resize(c->npairs);
+  c->h_pair_class.resize(c->npairs); c->h_pair_Q.resize(c->npairs);
+  for (int i = 0; i < c->npairs; ++i) {
+    c->h_pair_sa[i] = scr[i].a; c->h_pair_sb[i] = scr[i].b;
+    c->h_pair_class[i] = scr[i].cls; c->h_pair_Q[i] = scr[i].Q;
+  }
+  c->class_pair_off.assign(nclass, 0); c->class_pair_cnt.assign(nclass, 0);
+  for (int i = c->npairs - 1; i >= 0; --i) c->class_pair_off[scr[i].cls] = i;
+  for (int i = 0; i < c->npairs; ++i) c->class_pair_cnt[scr[i].cls]++;
+  // ---- quartet lists
+  c->work.clear();
+  c->nq_class.assign(QX_NUM_CLASSES, 0);
+  c->ndq_class.assign(QX_NUM_CLASSES, 0);
+  // positions (within their class) of pairs whose two shells sit on one atom,
+  // per (class, atom), to count single-atom quartets
+  std::vector<std::vector<std::vector<int>>> intra(nclass, std::vector<std::vector<int>>(c->natom));
+  for (int X = 0; X < nclass; ++X)
+    for (int i = 0; i < c->class_pair_cnt[X]; ++i) {
+      const int P = c->class_pair_off[X] + i;
+      const int A = c->atom[c->h_pair_sa[P]];
+      if (A == c->atom[c->h_pair_sb[P]]) intra[X][A].push_back(i);
+    }
+  std::vector<long long> allpre;
+  c->nquart_total = 0; c->npq_total = 0; c->store_total = 0;
+  for (int X = 0; X < nclass; ++X) {
+    for (int Y = 0; Y <= X; ++Y) {
+      const int bo = c->class_pair_off[X], bn = c->class_pair_cnt[X];
+      const int ko = c->class_pair_off[Y], kn = c->class_pair_cnt[Y];
+      if (bn == 0 || kn == 0) continue;
+      // cumulative pseudo counts over the ket class
+      std::vector<long long> kcum(kn + 1, 0);
+      for (int j = 0; j < kn; ++j) kcum[j + 1] = kcum[j] + c->h_pp_n[ko + j];
+      QxWorkHost wh{};
+      wh.cls = class_index(X, Y);
+      wh.bra_class = X; wh.ket_class = Y;
+      wh.prefix_off = (long long)allpre.size();
+      const int u0 = c->h_pp_first[bo];
+      const int u1 = c->h_pp_first[bo + bn - 1] + c->h_pp_n[bo + bn - 1];
+      allpre.push_back(0);
+      long long run = 0, ncq = 0;
+      for (int i = 0; i < bn; ++i) {
+        const double Qi = c->h_pair_Q[bo + i];
+        // kets with Q_i Q_j >= tol^2 form a prefix of the descending list
+        int lo = 0, hi = kn;
+        while (lo < hi) {
+          const int mid = (lo + hi) / 2;
+          if (Qi * c->h_pair_Q[ko + mid] >= tol2) lo = mid + 1; else hi = mid;
+        }
+        int kc = lo;
+        if (X == Y) kc = std::min(kc, i + 1);
+        ncq += kc;
+        {
+          const int P = bo + i;
+          const int A = c->atom[c->h_pair_sa[P]];
+          long long single = 0;
+          if (A == c->atom[c->h_pair_sb[P]]) {
+            const auto& v = intra[Y][A];
+            single = std::lower_bound(v.begin(), v.end(), kc) - v.begin();
+          }
+          c->ndq_class[class_index(X, Y)] += kc - single;
+        }
+        for (int k = 0; k < c->h_pp_n[bo + i]; ++k) {
+          run += kcum[kc];
+          allpre.push_back(run);
+        }
+      }
+      wh.w.bra_off = u0; wh.w.nbra = u1 - u0;
+      wh.w.ket_off = c->h_pp_first[ko];
+      wh.w.nket = (int)kcum[kn];
+      wh.w.nquart = run;
+      wh.w.diag = 0;
+      wh.w.store_base = c->store_total;
+      const int N = ncart(c->l[c->h_pair_sa[bo]]) * ncart(c->l[c->h_pair_sb[bo]]) *
+                    ncart(c->l[c->h_pair_sa[ko]]) * ncart(c->l[c->h_pair_sb[ko]]);
+      c->store_total += run * N;
+      c->nquart_total += ncq;
+      c->nq_class[class_index(X, Y)] += ncq;
+      c->npq_total += run;
+      if (run > 0) c->work.push_back(wh);
+      else allpre.resize(wh.prefix_off);
+    }
+  }
+  c->d_prefix = dupload(c, allpre);
+  for (auto& wh : c->work) wh.w.prefix = c->d_prefix + wh.prefix_off;
+}
+
+void setup_shell_tables(qx_ctx* c) {
+  QxShellsDev& s = c->shells;
+  s.nshell = c->nshell; s.natom = c->natom; s.ncart = c->ncart; s.nbf = c->nbf; s.pure = c->pure;
+  s.l = dupload(c, c->l); s.atom = dupload(c, c->atom); s.nprim = dupload(c, c->nprim);
+  s.poff = dupload(c, c->poff); s.bfc = dupload(c, c->bfc); s.bfs = dupload(c, c->bfs);
+  s.exps = dupload(c, c->exps); s.coefs = dupload(c, c->coefs);
+  s.xyz = dupload(c, c->xyz); s.Z = dupload(c, c->Z);
+  std::vector<int> fs, fc, as;
+  for (int sh = 0; sh < c->nshell; ++sh) {
+    const int l = c->l[sh];
+    for (int i = 0; i <= l; ++i)
+      for (int j = 0; j <= i; ++j) {
+        fs.push_back(sh);
+        fc.push_back(((l - i) << 16) | ((i - j) << 8) | j);
+      }
+    for (int k = 0; k < nsph(l, c->pure); ++k) as.push_back(sh);
+  }
+  s.f_shell = dupload(c, fs); s.f_comp = dupload(c, fc); s.ao_shell = dupload(c, as);
+  // transform blocks: identity for l < 2 (stored for uniform access), solid harmonics for d
+  std::vector<double> blk;
+  for (int l = 0; l <= QX_LMAX_SUPPORTED + 1; ++l) s.sph_off[l] = 0;
+  s.sph_off[0] = 0; blk.push_back(1.0);
+  s.sph_off[1] = (int)blk.size();
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) blk.push_back(i == j ? 1.0 : 0.0);
+  s.sph_off[2] = (int)blk.size();
+  if (c->pure) {
+    for (int k = 0; k < 30; ++k) blk.push_back(kSph2[k]);
+  } else {
+    for (int i = 0; i < 6; ++i)
+      for (int j = 0; j < 6; ++j) blk.push_back(i == j ? 1.0 : 0.0);
+  }
+  s.sph_off[3] = (int)blk.size();
+  s.sph = dupload(c, blk);
+}
+
+int fail(qx_ctx* c, const QxError& e) {
+  if (c) c->err = e.msg;
+  return e.code;
+}
+
+thread_local std::string g_init_error;
+
+}  // namespace
+
+// ============================================================== C ABI
+extern "C" {
+
+int qx_initialize(int natom, const double* xyz_bohr, const double* nuclear_charges, int nshell,
+                  const int* shell_l, const int* shell_atom, const int* shell_nprim, const double* exps,
+                  const double* coefs, int pure, double ints_tol, int device, qx_ctx** out) {
+  if (!out) return QX_EINVAL;
+  *out = nullptr;
+  qx_ctx* c = new qx_ctx();
+  try {
+    QX_REQUIRE(natom > 0 && xyz_bohr && nuclear_charges, QX_EINVAL, "natom must be positive with coordinates");
+    QX_REQUIRE(nshell > 0 && shell_l && shell_atom && shell_nprim && exps && coefs, QX_EINVAL, "no shells");
+    QX_REQUIRE(ints_tol > 0 && std::isfinite(ints_tol), QX_EINVAL, "ints_tol must be positive");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw QxError{QX_ENODEVICE, "no HIP device"};
+    QX_REQUIRE(device >= 0 && device < ndev, QX_EINVAL, "device index out of range");
+    QX_HIP(hipSetDevice(device));
+    c->device = device;
+    c->natom = natom;
+    c->xyz.assign(xyz_bohr, xyz_bohr + 3 * natom);
+    c->Z.assign(nuclear_charges, nuclear_charges + natom);
+    for (double v : c->xyz) QX_REQUIRE(std::isfinite(v), QX_EINVAL, "non-finite coordinate");
+    c->nshell = nshell;
+    c->pure = pure ? 1 : 0;
+    c->tol = ints_tol;
+    int tp = 0, nc = 0, nb = 0;
+    for (int s = 0; s < nshell; ++s) {
+      QX_REQUIRE(shell_l[s] >= 0 && shell_l[s] <= QX_LMAX_SUPPORTED, QX_EUNSUPPORTED,
+                 "shell angular momentum above " + std::to_string(QX_LMAX_SUPPORTED) + " is not built");
+      QX_REQUIRE(shell_atom[s] >= 0 && shell_atom[s] < natom, QX_EINVAL, "shell atom index out of range");
+      QX_REQUIRE(shell_nprim[s] >= 1, QX_EINVAL, "shell without primitives");
+      c->l.push_back(shell_l[s]);
+      c->atom.push_back(shell_atom[s]);
+      c->nprim.push_back(shell_nprim[s]);
+      c->poff.push_back(tp);
+      c->bfc.push_back(nc);
+      c->bfs.push_back(nb);
+      tp += shell_nprim[s];
+      nc += ncart(shell_l[s]);
+      nb += nsph(shell_l[s], c->pure);
+      if (shell_nprim[s] != 1) c->k1 = false;
+    }
+    c->ncart = nc;
+    c->nbf = nb;
+    c->exps.assign(exps, exps + tp);
+    c->coefs.resize(tp);
+    for (double v : c->exps) QX_REQUIRE(v > 0 && std::isfinite(v), QX_EINVAL, "exponents must be positive");
+    for (int s = 0; s < nshell; ++s)
+      normalize_shell(c->l[s], c->nprim[s], &c->exps[c->poff[s]], coefs + c->poff[s], &c->coefs[c->poff[s]]);
+    c->d_boys = dupload(c, boys_table());
+    setup_shell_tables(c);
+    schwarz_and_lists(c);
+    QX_HIP(hipDeviceSynchronize());
+  } catch (const QxError& e) {
+    g_init_error = e.msg;
+    int code = e.code;
+    qx_finalize(c);
+    return code;
+  }
+  *out = c;
+  return QX_OK;
+}
+
+int qx_finalize(qx_ctx* c) {
+  if (!c) return QX_EINVAL;
+  (void)hipSetDevice(c->device);
+  for (auto& p : c->prof_pending) {
+    (void)hipEventSynchronize(p.stop);
+    (void)hipEventDestroy(p.start);
+    (void)hipEventDestroy(p.stop);
+  }
+  for (auto e : c->event_pool) (void)hipEventDestroy(e);
+  for (void* p : c->allocs) (void)hipFree(p);
+  delete c;
+  return QX_OK;
+}
+
+const char* qx_last_error(const qx_ctx* c) { return c ? c->err.c_str() : g_init_error.c_str(); }
+
+int qx_nbf(const qx_ctx* c, int* nbf, int* nc) {
+  if (!c) return QX_EINVAL;
+  if (nbf) *nbf = c->nbf;
+  if (nc) *nc = c->ncart;
+  return QX_OK;
+}
+
+int qx_schwarz_screening(qx_ctx* c, double* values, int* pairs, int* npairs) {
+  if (!c) return QX_EINVAL;
+  if (values) std::memcpy(values, c->Q.data(), c->Q.size() * sizeof(double));
+  if (pairs) std::memcpy(pairs, c->ref_pairs.data(), c->ref_pairs.size() * sizeof(int));
+  if (npairs) *npairs = (int)c->ref_pairs.size() / 2;
+  return QX_OK;
+}
+
+int qx_num_quartets(const qx_ctx* c, long long* nq) {
+  if (!c || !nq) return QX_EINVAL;
+  *nq = c->nquart_total;
+  return QX_OK;
+}
+
+long long qx_how_many_derivs(int natom, int order) {
+  if (natom < 0 || order < 0) return -1;
+  long long val = 1, fact = 1;
+  for (int i = 0; i < order; ++i) {
+    val *= 3LL * natom + i;
+    fact *= i + 1;
+  }
+  return val / fact;
+}
+
+long long qx_generate_multi_index_lookup(int nparams, int order, int* out, long long cap) {
+  if (nparams < 0 || order < 0) return -1;
+  std::vector<int> cur(order, 0);
+  long long count = 0;
+  // combinations with replacement in lexicographic order
+  if (order == 0) return 1;
+  if (nparams == 0) return 0;
+  while (true) {
+    if (out) {
+      if (count >= cap) return -1;
+      for (int k = 0; k < order; ++k) out[count * order + k] = cur[k];
+    }
+    ++count;
+    int k = order - 1;
+    while (k >= 0 && cur[k] == nparams - 1) --k;
+    if (k < 0) break;
+    const int v = cur[k] + 1;
+    for (int j = k; j < order; ++j) cur[j] = v;
+  }
+  return count;
+}
+
+static void ao_tensor4_from_cart(qx_ctx* c, const double* cart, double* out, long long outer_slices, hipStream_t s) {
+  // four successive index transforms, Cartesian -> AO, per slice
+  const long long nc = c->ncart, nb = c->nbf;
+  DevBuf t1((size_t)outer_slices * nb * nc * nc * nc * sizeof(double));
+  DevBuf t2((size_t)outer_slices * nb * nb * nc * nc * sizeof(double));
+  check_launch(qxk_cart_to_ao_axis(&c->shells, cart, t1.d(), outer_slices, (int)nc, (int)nb, nc * nc * nc, s), "sph");
+  check_launch(qxk_cart_to_ao_axis(&c->shells, t1.d(), t2.d(), outer_slices * nb, (int)nc, (int)nb, nc * nc, s), "sph");
+  check_launch(qxk_cart_to_ao_axis(&c->shells, t2.d(), t1.d(), outer_slices * nb * nb, (int)nc, (int)nb, nc, s), "sph");
+  check_launch(qxk_cart_to_ao_axis(&c->shells, t1.d(), out, outer_slices * nb * nb * nb, (int)nc, (int)nb, 1, s), "sph");
+  QX_HIP(hipStreamSynchronize(s));
+}
+
+int qx_compute_2e_int(qx_ctx* c, double* out, void* stream) {
+  if (!c || !out) return QX_EINVAL;
+  try {
+    QX_HIP(hipSetDevice(c->device));
+    hipStream_t s = (hipStream_t)stream;
+    const size_t nc = c->ncart;
+    const size_t n4 = nc * nc * nc * nc;
+    DevBuf* tmp = c->pure ? new DevBuf(n4 * sizeof(double)) : nullptr;
+    std::unique_ptr<DevBuf> guard(tmp);
+    double* dst = c->pure ? tmp->d() : out;
+    check_launch(qxk_fill(dst, (long long)n4, 0.0, s), "fill");
+    QxEpiDev e{};
+    e.epi = QX_EPI_FULL; e.atomic = c->k1 ? 0 : 1; e.nbf = (int)nc; e.natom = c->natom; e.coord = -1; e.out = dst;
+    for (auto& wh : c->work) check_launch(launch_eri(c, wh.w, wh.cls, e, 0, s, c->pr, c->pp), "compute_2e_int");
+    if (c->pure) ao_tensor4_from_cart(c, dst, out, 1, s);
+  } catch (const QxError& e) {
+    return fail(c, e);
+  }
+  return QX_OK;
+}
+
+static void launch_deriv_all(qx_ctx* c, const QxEpiDev& e, int fp32, hipStream_t s) {
+  for (auto& wh : c->work) {
+    const QxClassEntry& ent = qx_class_table[wh.cls];
+    for (int k = 0; k < ent.nparts; ++k) {
+      QxArgs a;
+      a.pr = c->pr; a.pp = c->pp; a.w = wh.w; a.e = e; a.boys = c->d_boys;
+      QxLaunch L{fp32, s};
+      const int slot = QX_NUM_CLASSES + wh.cls * 12 + k;
+      check_launch(timed_launch(c, slot, wh.w.nquart, s, [&] { return ent.deri[k](&a, &L); }), "eri derivative");
+    }
+  }
+}
+
+static void deriv_tensor(qx_ctx* c, int coord, double* out, hipStream_t s) {
+  const size_t nc = c->ncart;
+  const size_t n4 = nc * nc * nc * nc;
+  const long long nslices = coord >= 0 ? 1 : 3LL * c->natom;
+  std::unique_ptr<DevBuf> tmp(c->pure ? new DevBuf(nslices * n4 * sizeof(double)) : nullptr);
+  double* dst = c->pure ? tmp->d() : out;
+  check_launch(qxk_fill(dst, (long long)(nslices * n4), 0.0, s), "fill");
+  QxEpiDev e{};
+  e.epi = QX_EPI_DFULL; e.atomic = 1; e.nbf = (int)nc; e.natom = c->natom; e.coord = coord; e.out = dst;
+  launch_deriv_all(c, e, 0, s);
+  if (c->pure) ao_tensor4_from_cart(c, dst, out, nslices, s);
+}
+
+int qx_eri_deriv_core(qx_ctx* c, int order, double* out, void* stream) {
+  if (!c || !out) return QX_EINVAL;
+  if (order != 1) {
+    c->err = "eri_deriv_core: only deriv_order 1 is built";
+    return QX_EUNSUPPORTED;
+  }
+  try {
+    QX_HIP(hipSetDevice(c->device));
+    deriv_tensor(c, -1, out, (hipStream_t)stream);
+  } catch (const QxError& e) {
+    return fail(c, e);
+  }
+  return QX_OK;
+}
+
+int qx_compute_2e_deriv(qx_ctx* c, const int* dv, int len, double* out, void* stream) {
+  if (!c || !out || !dv) return QX_EINVAL;
+  if (len != 3 * c->natom) {
+    c->err = "compute_2e_deriv: deriv_vec length must be 3*natom";
+    return QX_EINVAL;
+  }
+  int order = 0, coord = -1;
+  for (int i = 0; i < len; ++i) {
+    if (dv[i] < 0) { c->err = "compute_2e_deriv: negative deriv_vec entry"; return QX_EINVAL; }
+    order += dv[i];
+    if (dv[i]) coord = i;
+  }
+  if (order > 1) {
+    c->err = "compute_2e_deriv: only first-order deriv_vec is built";
+    return QX_EUNSUPPORTED;
+  }
+  try {
+    QX_HIP(hipSetDevice(c->device));
+    hipStream_t s = (hipStream_t)stream;
+    if (order == 0) {
+      // the reference's order-0 call returns buffer 0 of a deriv-0 engine, i.e.
+      // the integrals, with single-atom quartets skipped (libint_interface.cc:1180-1185)
+      return QX_EUNSUPPORTED;
+    }
+    deriv_tensor(c, coord, out, s);
+  } catch (const QxError& e) {
+    return fail(c, e);
+  }
+  return QX_OK;
+}
+
+int qx_jk_build(qx_ctx* c, const double* D, double* J, double* K, int precision, double thr, void* stream) {
+  if (!c || !D || !J || !K) return QX_EINVAL;
+  if (precision != QX_PREC_FP64 && precision != QX_PREC_MIXED) return QX_EINVAL;
+  try {
+    QX_HIP(hipSetDevice(c->device));
+    hipStream_t s = (hipStream_t)stream;
+    const size_t nc = c->ncart;
+    DevBuf Jp(nc * nc * sizeof(double)), Kp(nc * nc * sizeof(double));
+    std::unique_ptr<DevBuf> Dc(c->pure ? new DevBuf(nc * nc * sizeof(double)) : nullptr);
+    const double* Dcart = D;
+    if (c->pure) {
+      check_launch(qxk_ao_to_cart_mat(&c->shells, D, Dc->d(), s), "ao_to_cart");
+      Dcart = Dc->d();
+    }
+    check_launch(qxk_fill(Jp.d(), (long long)(nc * nc), 0.0, s), "fill");
+    check_launch(qxk_fill(Kp.d(), (long long)(nc * nc), 0.0, s), "fill");
+    QxEpiDev e{};
+    e.epi = QX_EPI_JK; e.nbf = (int)nc; e.natom = c->natom; e.coord = -1; e.D = Dcart; e.J = Jp.d(); e.K = Kp.d();
+    if (precision == QX_PREC_MIXED) {
+      c->err = "mixed precision J/K is not built";
+      return QX_EUNSUPPORTED;
+    }
+    for (auto& wh : c->work) check_launch(launch_eri(c, wh.w, wh.cls, e, 0, s, c->pr, c->pp), "jk_build");
+    if (c->pure) {
+      DevBuf Js(nc * nc * sizeof(double)), Ks(nc * nc * sizeof(double));
+      check_launch(qxk_sym_half(Jp.d(), Js.d(), (int)nc, s), "sym");
+      check_launch(qxk_sym_half(Kp.d(), Ks.d(), (int)nc, s), "sym");
+      check_launch(qxk_cart_to_ao_mat(&c->shells, Js.d(), J, s), "cart_to_ao");
+      check_launch(qxk_cart_to_ao_mat(&c->shells, Ks.d(), K, s), "cart_to_ao");
+      QX_HIP(hipStreamSynchronize(s));
+    } else {
+      check_launch(qxk_sym_half(Jp.d(), J, (int)nc, s), "sym");
+      check_launch(qxk_sym_half(Kp.d(), K, (int)nc, s), "sym");
+      QX_HIP(hipStreamSynchronize(s));
+    }
+  } catch (const QxError& e) {
+    return fail(c, e);
+  }
+  return QX_OK;
+}
+
+int qx_eri_grad(qx_ctx* c, const double* D, double* grad, int precision, double thr, void* stream) {
+  if (!c || !D || !grad) return QX_EINVAL;
+  if (precision != QX_PREC_FP64 && precision != QX_PREC_MIXED) return QX_EINVAL;
+  try {
+    QX_HIP(hipSetDevice(c->device));
+    hipStream_t s = (hipStream_t)stream;
+    const size_t nc = c->ncart;
+    std::unique_ptr<DevBuf> Dc(c->pure ? new DevBuf(nc * nc * sizeof(double)) : nullptr);
+    const double* Dcart = D;
+    if (c->pure) {
+      check_launch(qxk_ao_to_cart_mat(&c->shells, D, Dc->d(), s), "ao_to_cart");
+      Dcart = Dc->d();
+    }
+    if (precision == QX_PREC_MIXED) {
+      c->err = "mixed precision gradient is not built";
+      return QX_EUNSUPPORTED;
+    }
+    QxEpiDev e{};
+    e.epi = QX_EPI_GRAD; e.nbf = (int)nc; e.natom = c->natom; e.coord = -1; e.D = Dcart; e.grad = grad;
+    launch_deriv_all(c, e, 0, s);
+    QX_HIP(hipStreamSynchronize(s));
+  } catch (const QxError& e) {
+    return fail(c, e);
+  }
+  return QX_OK;
+}
+
+int qx_oei(qx_ctx* c, double* S, double* T, double* V, void* stream) {
+  if (!c || !S || !T || !V) return QX_EINVAL;
+  try {
+    QX_HIP(hipSetDevice(c->device));
+    hipStream_t s = (hipStream_t)stream;
+    const size_t nc = c->ncart;
+    if (!c->pure) {
+      check_launch(qxk_oei(&c->shells, S, T, V, s), "oei");
+    } else {
+      DevBuf a(nc * nc * 8), b(nc * nc * 8), d(nc * nc * 8);
+      check_launch(qxk_oei(&c->shells, a.d(), b.d(), d.d(), s), "oei");
+      check_launch(qxk_cart_to_ao_mat(&c->shells, a.d(), S, s), "cart_to_ao");
+      check_launch(qxk_cart_to_ao_mat(&c->shells, b.d(), T, s), "cart_to_ao");
+      check_launch(qxk_cart_to_ao_mat(&c->shells, d.d(), V, s), "cart_to_ao");
+      QX_HIP(hipStreamSynchronize(s));
+    }
+  } catch (const QxError& e) {
+    return fail(c, e);
+  }
+  return QX_OK;
+}
+
+int qx_oei_grad(qx_ctx* c, const double* Pd, const double* Wd, double* grad, void* stream) {
+  if (!c || !Pd || !Wd || !grad) return QX_EINVAL;
+  try {
+    QX_HIP(hipSetDevice(c->device));
+    hipStream_t s = (hipStream_t)stream;
+    const size_t nc = c->ncart;
+    if (!c->pure) {
+      check_launch(qxk_oei_deriv(&c->shells, Pd, Wd, grad, nullptr, nullptr, nullptr, s), "oei_grad");
+    } else {
+      DevBuf a(nc * nc * 8), b(nc * nc * 8);
+      check_launch(qxk_ao_to_cart_mat(&c->shells, Pd, a.d(), s), "ao_to_cart");
+      check_launch(qxk_ao_to_cart_mat(&c->shells, Wd, b.d(), s), "ao_to_cart");
+      check_launch(qxk_oei_deriv(&c->shells, a.d(), b.d(), grad, nullptr, nullptr, nullptr, s), "oei_grad");
+      QX_HIP(hipStreamSynchronize(s));
+    }
+  } catch (const QxError& e) {
+    return fail(c, e);
+  }
+  return QX_OK;
+}
+
+int qx_oei_deriv(qx_ctx* c, double* dS, double* dT, double* dV, void* stream) {
+  if (!c || !dS || !dT || !dV) return QX_EINVAL;
+  if (c->pure) {
+    c->err = "oei_deriv: Cartesian bases only";
+    return QX_EUNSUPPORTED;
+  }
+  try {
+    QX_HIP(hipSetDevice(c->device));
+    hipStream_t s = (hipStream_t)stream;
+    const long long n = 3LL * c->natom * c->ncart * c->ncart;
+    check_launch(qxk_fill(dS, n, 0.0, s), "fill");
+    check_launch(qxk_fill(dT, n, 0.0, s), "fill");
+    check_launch(qxk_fill(dV, n, 0.0, s), "fill");
+    check_launch(qxk_oei_deriv(&c->shells, nullptr, nullptr, nullptr, dS, dT, dV, s), "oei_deriv");
+  } catch (const QxError& e) {
+    return fail(c, e);
+  }
+  return QX_OK;
+}
+
+int qx_store_size(const qx_ctx* c, long long* nd) {
+  if (!c || !nd) return QX_EINVAL;
+  *nd = c->store_total;
+  return QX_OK;
+}
+
+int qx_eri_store(qx_ctx* c, double* out, void* stream) {
+  if (!c || !out) return QX_EINVAL;
+  if (!c->k1) {
+    c->err = "eri_store: single-primitive shells only";
+    return QX_EUNSUPPORTED;
+  }
+  try {
+    QX_HIP(hipSetDevice(c->device));
+    hipStream_t s = (hipStream_t)stream;
+    QxEpiDev e{};
+    e.epi = QX_EPI_STORE; e.nbf = c->ncart; e.natom = c->natom; e.coord = -1; e.out = out;
+    for (auto& wh : c->work) check_launch(launch_eri(c, wh.w, wh.cls, e, 0, s, c->pr, c->pp), "eri_store");
+  } catch (const QxError& e) {
+    return fail(c, e);
+  }
+  return QX_OK;
+}
+
+int qx_num_classes(void) { return QX_NUM_CLASSES; }
+
+int qx_profile_slots(void) { return QX_PROFILE_SLOTS; }
+
+int qx_class_info(int cls, int* l4, int* nparts) {
+  if (cls < 0 || cls >= QX_NUM_CLASSES) return QX_EINVAL;
+  const QxClassEntry& e = qx_class_table[cls];
+  if (l4) { l4[0] = e.la; l4[1] = e.lb; l4[2] = e.lc; l4[3] = e.ld; }
+  if (nparts) *nparts = e.nparts;
+  return QX_OK;
+}
+
+int qx_class_flops(int cls, double* eri, double* parts, int* ncomb) {
+  if (cls < 0 || cls >= QX_NUM_CLASSES) return QX_EINVAL;
+  const QxClassEntry& e = qx_class_table[cls];
+  if (eri) *eri = e.eri_flops;
+  for (int k = 0; k < e.nparts; ++k) {
+    if (parts) parts[k] = e.part_flops[k];
+    if (ncomb) ncomb[k] = e.part_ncomb[k];
+  }
+  return QX_OK;
+}
+
+int qx_class_quartets(const qx_ctx* c, long long* eri, long long* deri) {
+  if (!c) return QX_EINVAL;
+  for (int k = 0; k < QX_NUM_CLASSES; ++k) {
+    if (eri) eri[k] = c->nq_class[k];
+    if (deri) deri[k] = c->ndq_class[k];
+  }
+  return QX_OK;
+}
+
+int qx_profile_enable(qx_ctx* c, int on) {
+  if (!c) return QX_EINVAL;
+  try {
+    double tmp[QX_PROFILE_SLOTS];
+    long long t1[QX_PROFILE_SLOTS], t2[QX_PROFILE_SLOTS];
+    if (!c->prof_pending.empty()) qx_profile_read(c, tmp, t1, t2);
+    c->profiling = on != 0;
+    c->prof_ms.assign(QX_PROFILE_SLOTS, 0.0);
+    c->prof_launches.assign(QX_PROFILE_SLOTS, 0);
+    c->prof_pq.assign(QX_PROFILE_SLOTS, 0);
+  } catch (const QxError& e) {
+    return fail(c, e);
+  }
+  return QX_OK;
+}
+
+int qx_profile_read(qx_ctx* c, double* ms, long long* launches, long long* pq) {
+  if (!c) return QX_EINVAL;
+  try {
+    if (c->prof_ms.size() != QX_PROFILE_SLOTS) {
+      c->prof_ms.assign(QX_PROFILE_SLOTS, 0.0);
+      c->prof_launches.assign(QX_PROFILE_SLOTS, 0);
+      c->prof_pq.assign(QX_PROFILE_SLOTS, 0);
+    }
+    for (auto& p : c->prof_pending) {
+      QX_HIP(hipEventSynchronize(p.stop));
+      float t = 0.0f;
+      QX_HIP(hipEventElapsedTime(&t, p.start, p.stop));
+      c->prof_ms[p.slot] += t;
+      c->event_pool.push_back(p.start);
+      c->event_pool.push_back(p.stop);
+    }
+    c->prof_pending.clear();
+    for (int k = 0; k < QX_PROFILE_SLOTS; ++k) {
+      if (ms) ms[k] = c->prof_ms[k];
+      if (launches) launches[k] = c->prof_launches[k];
+      if (pq) pq[k] = c->prof_pq[k];
+    }
+  } catch (const QxError& e) {
+    return fail(c, e);
+  }
+  return QX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,192 @@
+"""Device-resident ERI engine: one molecule + basis on one GPU.
+
+Wraps a ``qx_ctx`` of libquax_hip.so.  Buffers are torch tensors on the
+engine's device; every call goes through the C ABI and runs hand-written HIP
+kernels.  There is no CPU fallback: without the library or a GPU the
+constructor raises.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import QX_PREC_FP64, QX_PREC_MIXED, NativeUnavailable, check, ptr
+
+
+def _require_gpu():
+    import torch
+    if not torch.cuda.is_available():
+        raise NativeUnavailable("quax_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
+    return torch
+
+
+class EriEngine:
+    def __init__(self, molecule, basis, ints_tol: float = 1e-14, device: int = 0):
+        torch = _require_gpu()
+        self.torch = torch
+        self.L = _lib.load()
+        self.molecule = molecule
+        self.basis = basis
+        self.device = torch.device("cuda", device)
+        self.natom = molecule.natom()
+        xyz = np.ascontiguousarray(molecule.geometry().reshape(-1), dtype=np.float64)
+        Z = np.array([molecule.charge(i) for i in range(self.natom)], dtype=np.float64)
+        l, atom, nprim, exps, coefs = basis.arrays()
+        self._keep = (xyz, Z, l, atom, nprim, exps, coefs)
+        ctx = ctypes.c_void_p()
+        dp = ctypes.POINTER(ctypes.c_double)
+        ip = ctypes.POINTER(ctypes.c_int)
+        with torch.cuda.device(self.device):
+            torch.cuda.current_stream().synchronize()
+            rc = self.L.qx_initialize(self.natom, xyz.ctypes.data_as(dp), Z.ctypes.data_as(dp), len(l),
+                                      l.ctypes.data_as(ip), atom.ctypes.data_as(ip), nprim.ctypes.data_as(ip),
+                                      exps.ctypes.data_as(dp), coefs.ctypes.data_as(dp), 1 if basis.pure else 0,
+                                      float(ints_tol), device, ctypes.byref(ctx))
+        if rc != 0:
+            raise _lib.QuaxNativeError(rc, "qx_initialize: " + self.L.qx_last_error(None).decode())
+        self.ctx = ctx
+        nbf, nc = ctypes.c_int(), ctypes.c_int()
+        check(self.L.qx_nbf(self.ctx, ctypes.byref(nbf), ctypes.byref(nc)), self.ctx)
+        self.nbf, self.ncart = nbf.value, nc.value
+        self.ints_tol = ints_tol
+
+    # ------------------------------------------------------------ lifetime
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.L.qx_finalize(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ------------------------------------------------------------ helpers
+    def _empty(self, *shape):
+        return self.torch.empty(shape, dtype=self.torch.float64, device=self.device)
+
+    def _zeros(self, *shape):
+        return self.torch.zeros(shape, dtype=self.torch.float64, device=self.device)
+
+    def _stream(self):
+        return ctypes.c_void_p(self.torch.cuda.current_stream(self.device).cuda_stream)
+
+    def _dev(self, x):
+        t = self.torch.as_tensor(x, dtype=self.torch.float64, device=self.device)
+        return t.contiguous()
+
+    # ------------------------------------------------------------ queries
+    def schwarz_screening(self):
+        ns = self.basis.nshell()
+        vals = np.zeros((ns, ns))
+        pairs = np.zeros(ns * (ns + 1), dtype=np.int32)
+        npairs = ctypes.c_int()
+        check(self.L.qx_schwarz_screening(self.ctx, vals.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                          pairs.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                          ctypes.byref(npairs)), self.ctx)
+        return vals, pairs[:2 * npairs.value].reshape(-1, 2)
+
+    def num_quartets(self) -> int:
+        n = ctypes.c_longlong()
+        check(self.L.qx_num_quartets(self.ctx, ctypes.byref(n)), self.ctx)
+        return n.value
+
+    def class_quartets(self):
+        """Per kernel class: ((la,lb,lc,ld), quartets, non-single-atom quartets, derivative parts)."""
+        nc = self.L.qx_num_classes()
+        eri = (ctypes.c_longlong * nc)()
+        deri = (ctypes.c_longlong * nc)()
+        check(self.L.qx_class_quartets(self.ctx, eri, deri), self.ctx)
+        out = []
+        for k in range(nc):
+            l4 = (ctypes.c_int * 4)()
+            npart = ctypes.c_int()
+            self.L.qx_class_info(k, l4, ctypes.byref(npart))
+            out.append((tuple(l4), eri[k], deri[k], npart.value))
+        return out
+
+    def profile(self, on: bool):
+        check(self.L.qx_profile_enable(self.ctx, 1 if on else 0), self.ctx)
+
+    def profile_read(self):
+        """(ms, launches, primitive quartets) per profile slot."""
+        ns = self.L.qx_profile_slots()
+        ms = (ctypes.c_double * ns)()
+        la = (ctypes.c_longlong * ns)()
+        pq = (ctypes.c_longlong * ns)()
+        check(self.L.qx_profile_read(self.ctx, ms, la, pq), self.ctx)
+        return np.array(ms[:]), np.array(la[:]), np.array(pq[:])
+
+    # ------------------------------------------------------------ tensors
+    def compute_2e_int(self):
+        n = self.nbf
+        out = self._empty(n, n, n, n)
+        check(self.L.qx_compute_2e_int(self.ctx, ptr(out), self._stream()), self.ctx, "compute_2e_int")
+        return out
+
+    def eri_deriv_core(self, deriv_order: int = 1):
+        n = self.nbf
+        nu = self.L.qx_how_many_derivs(self.natom, deriv_order)
+        out = self._empty(nu, n, n, n, n)
+        check(self.L.qx_eri_deriv_core(self.ctx, int(deriv_order), ptr(out), self._stream()), self.ctx,
+              "eri_deriv_core")
+        return out
+
+    def compute_2e_deriv(self, deriv_vec):
+        dv = np.ascontiguousarray(np.asarray(deriv_vec).reshape(-1), dtype=np.int32)
+        n = self.nbf
+        out = self._empty(n, n, n, n)
+        check(self.L.qx_compute_2e_deriv(self.ctx, dv.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), len(dv),
+                                         ptr(out), self._stream()), self.ctx, "compute_2e_deriv")
+        return out
+
+    def store(self):
+        nd = ctypes.c_longlong()
+        check(self.L.qx_store_size(self.ctx, ctypes.byref(nd)), self.ctx)
+        out = self._empty(nd.value)
+        check(self.L.qx_eri_store(self.ctx, ptr(out), self._stream()), self.ctx, "eri_store")
+        return out
+
+    # ------------------------------------------------------------ direct
+    def jk(self, D, precision: int = QX_PREC_FP64, fp32_threshold: float = 0.0, out=None):
+        D = self._dev(D)
+        n = self.nbf
+        J, K = (out if out is not None else (self._empty(n, n), self._empty(n, n)))
+        check(self.L.qx_jk_build(self.ctx, ptr(D), ptr(J), ptr(K), int(precision), float(fp32_threshold),
+                                 self._stream()), self.ctx, "jk_build")
+        return J, K
+
+    def eri_grad(self, D, precision: int = QX_PREC_FP64, fp32_threshold: float = 0.0):
+        D = self._dev(D)
+        g = self._zeros(3 * self.natom)
+        check(self.L.qx_eri_grad(self.ctx, ptr(D), ptr(g), int(precision), float(fp32_threshold), self._stream()),
+              self.ctx, "eri_grad")
+        return g
+
+    def oei(self):
+        n = self.nbf
+        S, T, V = self._empty(n, n), self._empty(n, n), self._empty(n, n)
+        check(self.L.qx_oei(self.ctx, ptr(S), ptr(T), ptr(V), self._stream()), self.ctx, "oei")
+        return S, T, V
+
+    def oei_grad(self, Pd, Wd):
+        Pd, Wd = self._dev(Pd), self._dev(Wd)
+        g = self._zeros(3 * self.natom)
+        check(self.L.qx_oei_grad(self.ctx, ptr(Pd), ptr(Wd), ptr(g), self._stream()), self.ctx, "oei_grad")
+        return g
+
+    def oei_deriv(self):
+        n = self.nbf
+        shp = (3 * self.natom, n, n)
+        dS, dT, dV = self._empty(*shp), self._empty(*shp), self._empty(*shp)
+        check(self.L.qx_oei_deriv(self.ctx, ptr(dS), ptr(dT), ptr(dV), self._stream()), self.ctx, "oei_deriv")
+        return dS, dT, dV
