@@ -262,8 +262,11 @@ def rhf(S, T, V, G, nelec, enuc, maxit=100, spectral_shift=False, conv=1e-12, da
         D_old = D.copy()
         J, K = jkf(D)
         F = H + 2 * J - K
-        E_new, D, C, eps = it(F, D)
+        # orbital gradient of the density that built F.  (The reference
+        # evaluates it with the NEW density, which commutes with F by
+        # construction, so only |dE| gates its convergence.)
         err = A @ (F @ D @ S - S @ D @ F) @ A
+        E_new, D, C, eps = it(F, D)
         drms = np.sqrt(np.mean(err ** 2))
         dE = E - E_new
         E = E_new

@@ -63,6 +63,7 @@ struct QxWorkDev {         // one (bra class, ket class) quartet list
   int bra_off, nbra;         // bra pseudo-pair range
   int ket_off, nket;         // ket pseudo-pair range
   const long long* prefix;   // nbra+1 running quartet counts
+  const int* kstart;         // per bra pseudo pair: first admissible ket (null = 0)
   long long nquart;
   long long store_base;      // element offset of this list in the store buffer
   int diag;                  // kets are the pseudo pairs of the bra's own pair
@@ -134,11 +135,34 @@ __device__ __forceinline__ void qx_locate(const QxArgs& A_, long long t, int& u1
   const int local = (int)(t - pre[lo]);
   u1 = A_.w.bra_off + lo;
   P1 = A_.pp.pair[u1];
-  u2 = A_.w.diag ? A_.pr.pp_first[P1] + local : A_.w.ket_off + local;
+  u2 = A_.w.diag ? A_.pr.pp_first[P1] + local : A_.w.ket_off + (A_.w.kstart ? A_.w.kstart[lo] : 0) + local;
   P2 = A_.pp.pair[u2];
 }
 
 __device__ __forceinline__ void qx_atomic_add(double* p, double v) { unsafeAtomicAdd(p, v); }
+
+// true when every active lane of the wavefront holds the same key
+__device__ __forceinline__ bool qx_wave_uniform(int key, bool active) {
+  int kmin = active ? key : 0x7fffffff, kmax = active ? key : (int)0x80000000;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    kmin = min(kmin, __shfl_xor(kmin, off));
+    kmax = max(kmax, __shfl_xor(kmax, off));
+  }
+  return kmin == kmax;
+}
+
+__device__ __forceinline__ double qx_wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+// lane that performs a wave-reduced store: the first active lane
+__device__ __forceinline__ bool qx_wave_leader(bool active) {
+  const unsigned long long m = __ballot(active);
+  return m != 0 && (int)__lane_id() == __ffsll((long long)m) - 1;
+}
 
 __device__ __forceinline__ void qx_put(double* p, double v, bool atomic) {
   if (atomic) qx_atomic_add(p, v); else *p = v;
@@ -174,9 +198,20 @@ __device__ __forceinline__ void qx_scatter8(double* out, size_t n, size_t I, siz
 
 // ---- ERI epilogues ----------------------------------------------------
 template <int NA, int NB, int NC, int ND, typename R>
-__device__ __forceinline__ void qx_epi_jk(const QxArgs& A_, int P1, int P2, const R (&o)[NA * NB * NC * ND]) {
+__device__ __forceinline__ void qx_epi_jk(const QxArgs& A_, int P1, int P2, const R (&o)[NA * NB * NC * ND],
+                                          bool active) {
   // J'_ab += 4f (ab|cd) D_cd, J'_cd += 4f (ab|cd) D_ab,
-  // K'_ac += 2f (ab|cd) D_bd, K'_ad += 2f D_bc, K'_bc += 2f D_ad, K'_bd += 2f D_ac
+  // K'_ac += 2f (ab|cd) D_bd, K'_ad += 2f D_bc, K'_bc += 2f D_ad, K'_bd += 2f D_ac.
+  // Lanes of one wavefront usually share the bra pair (consecutive kets of
+  // one bra row): then J'_ab is summed across the wavefront and added once.
+  const bool uni = qx_wave_uniform(P1, active);
+  if (!active) {
+    if (uni) {
+#pragma unroll
+      for (int k = 0; k < NA * NB; ++k) (void)qx_wave_sum(0.0);
+    }
+    return;
+  }
   const int n = A_.e.nbf;
   const int ia = A_.pr.bfa[P1], ib = A_.pr.bfb[P1], ic = A_.pr.bfa[P2], id = A_.pr.bfb[P2];
   const double* D = A_.e.D;
@@ -223,10 +258,19 @@ __device__ __forceinline__ void qx_epi_jk(const QxArgs& A_, int P1, int P2, cons
   const double fj = 4.0 * f, fk = 2.0 * f;
   double* J = A_.e.J;
   double* K = A_.e.K;
+  if (uni) {
+    const bool lead = qx_wave_leader(true);
 #pragma unroll
-  for (int i = 0; i < NA; ++i)
+    for (int k = 0; k < NA * NB; ++k) {
+      const double v = qx_wave_sum(fj * jab[k]);
+      if (lead) qx_atomic_add(&J[(ia + k / NB) * n + ib + k % NB], v);
+    }
+  } else {
 #pragma unroll
-    for (int j = 0; j < NB; ++j) qx_atomic_add(&J[(ia + i) * n + ib + j], fj * jab[i * NB + j]);
+    for (int i = 0; i < NA; ++i)
+#pragma unroll
+      for (int j = 0; j < NB; ++j) qx_atomic_add(&J[(ia + i) * n + ib + j], fj * jab[i * NB + j]);
+  }
 #pragma unroll
   for (int k = 0; k < NC; ++k)
 #pragma unroll
@@ -249,12 +293,15 @@ __device__ __forceinline__ void qx_epi_jk(const QxArgs& A_, int P1, int P2, cons
 
 template <int NA, int NB, int NC, int ND, typename R>
 __device__ __forceinline__ void qx_epi_eri(const QxArgs& A_, long long t, int P1, int P2,
-                                           const R (&o)[NA * NB * NC * ND]) {
+                                           const R (&o)[NA * NB * NC * ND], bool active) {
   constexpr int N = NA * NB * NC * ND;
   const int epi = A_.e.epi;
   if (epi == QX_EPI_JK) {
-    qx_epi_jk<NA, NB, NC, ND, R>(A_, P1, P2, o);
-  } else if (epi == QX_EPI_STORE) {
+    qx_epi_jk<NA, NB, NC, ND, R>(A_, P1, P2, o, active);
+    return;
+  }
+  if (!active) return;
+  if (epi == QX_EPI_STORE) {
     double* out = A_.e.out + A_.w.store_base + t;
     const long long nq = A_.w.nquart;
 #pragma unroll
@@ -293,7 +340,8 @@ __device__ __forceinline__ bool qx_single_atom(const QxArgs& A_, int P1, int P2)
 }
 
 template <int NA, int NB, int NC, int ND, int CMASK, int XMASK, int DINV, int NCOMB, typename R>
-__device__ __forceinline__ void qx_epi_deri(const QxArgs& A_, int P1, int P2, const R (&dv)[NCOMB * NA * NB * NC * ND]) {
+__device__ __forceinline__ void qx_epi_deri(const QxArgs& A_, int P1, int P2, const R (&dv)[NCOMB * NA * NB * NC * ND],
+                                            double (&gv)[12]) {
   constexpr int N = NA * NB * NC * ND;
   const int at[4] = {A_.pr.atom_a[P1], A_.pr.atom_b[P1], A_.pr.atom_a[P2], A_.pr.atom_b[P2]};
   const int n = A_.e.nbf;
@@ -323,24 +371,24 @@ __device__ __forceinline__ void qx_epi_deri(const QxArgs& A_, int P1, int P2, co
           }
         }
       }
+    // per (centre, direction) contributions, returned to the kernel which
+    // accumulates them per block
     const double w = 8.0 * qx_degeneracy(A_, P1, P2);
-    double* G = A_.e.grad;
     int ci = 0;
     double dsum[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int cen = 0; cen < 4; ++cen) {
-      if (!(CMASK & (1 << cen))) continue;
 #pragma unroll
       for (int x = 0; x < 3; ++x) {
-        if (!(XMASK & (1 << x))) continue;
-        const double v = w * g[ci++];
+        double v = 0.0;
+        if ((CMASK & (1 << cen)) && (XMASK & (1 << x))) v = w * g[ci++];
         dsum[x] += v;
-        qx_atomic_add(&G[3 * at[cen] + x], v);
+        gv[3 * cen + x] = v;
       }
     }
     if (DINV) {
 #pragma unroll
-      for (int x = 0; x < 3; ++x) qx_atomic_add(&G[3 * at[3] + x], -dsum[x]);
+      for (int x = 0; x < 3; ++x) gv[9 + x] = -dsum[x];
     }
   } else {  // QX_EPI_DFULL
     const size_t nn = n;
@@ -388,16 +436,18 @@ __device__ __forceinline__ void qx_epi_deri(const QxArgs& A_, int P1, int P2, co
 // ---- kernel definitions (instantiated once per generated file) --------
 #define QX_GRID(nq) dim3((unsigned)(((nq) + QX_BLOCK - 1) / QX_BLOCK))
 
+#define QX_GRAD_LDS 768  // block-level gradient accumulators (3 * atoms)
+
 #define QX_DEFINE_ERI_KERNEL(NAME, N, NA, NB, NC, ND, WITH32)                                     \
   template <typename R>                                                                           \
   __global__ __launch_bounds__(QX_BLOCK) void qx_k_##NAME(QxArgs A_) {                            \
     const long long t = (long long)blockIdx.x * QX_BLOCK + threadIdx.x;                          \
-    if (t >= A_.w.nquart) return;                                                                 \
-    int u1, u2, P1, P2;                                                                           \
-    qx_locate(A_, t, u1, u2, P1, P2);                                                             \
+    const bool active = t < A_.w.nquart;                                                          \
+    int u1 = 0, u2 = 0, P1 = -1, P2 = -1;                                                         \
+    if (active) qx_locate(A_, t, u1, u2, P1, P2);                                                 \
     R o[N];                                                                                       \
-    qx_core_##NAME<R>(A_, P1, P2, u1, u2, o);                                                     \
-    qx_epi_eri<NA, NB, NC, ND, R>(A_, t, P1, P2, o);                                              \
+    if (active) qx_core_##NAME<R>(A_, P1, P2, u1, u2, o);                                         \
+    qx_epi_eri<NA, NB, NC, ND, R>(A_, t, P1, P2, o, active);                                      \
   }                                                                                               \
   extern "C" int qx_launch_##NAME(const QxArgs* a, const QxLaunch* L) {                           \
     if (a->w.nquart <= 0) return 0;                                                               \
@@ -411,17 +461,44 @@ __device__ __forceinline__ void qx_epi_deri(const QxArgs& A_, int P1, int P2, co
     return (int)hipGetLastError();                                                                \
   }
 
+// The gradient epilogue accumulates into LDS (one slot per nuclear
+// coordinate) and flushes once per block: FP64 atomics on a few dozen global
+// addresses from every quartet would serialise.
 #define QX_DEFINE_DERI_KERNEL(NAME, N, NA, NB, NC, ND, CMASK, XMASK, DINV, NCOMB, WITH32)         \
   template <typename R>                                                                           \
   __global__ __launch_bounds__(QX_BLOCK) void qx_k_##NAME(QxArgs A_) {                            \
+    __shared__ double sg[QX_GRAD_LDS];                                                            \
+    const int ng = 3 * A_.e.natom;                                                                \
+    const bool lds = A_.e.epi == QX_EPI_GRAD && ng <= QX_GRAD_LDS;                                \
+    if (lds) {                                                                                    \
+      for (int i = threadIdx.x; i < ng; i += QX_BLOCK) sg[i] = 0.0;                               \
+      __syncthreads();                                                                            \
+    }                                                                                             \
     const long long t = (long long)blockIdx.x * QX_BLOCK + threadIdx.x;                          \
-    if (t >= A_.w.nquart) return;                                                                 \
-    int u1, u2, P1, P2;                                                                           \
-    qx_locate(A_, t, u1, u2, P1, P2);                                                             \
-    if (qx_single_atom(A_, P1, P2)) return;                                                       \
-    R dv[NCOMB * N];                                                                              \
-    qx_core_##NAME<R>(A_, P1, P2, u1, u2, dv);                                                    \
-    qx_epi_deri<NA, NB, NC, ND, CMASK, XMASK, DINV, NCOMB, R>(A_, P1, P2, dv);                    \
+    if (t < A_.w.nquart) {                                                                        \
+      int u1, u2, P1, P2;                                                                         \
+      qx_locate(A_, t, u1, u2, P1, P2);                                                           \
+      if (!qx_single_atom(A_, P1, P2)) {                                                          \
+        R dv[NCOMB * N];                                                                          \
+        qx_core_##NAME<R>(A_, P1, P2, u1, u2, dv);                                                \
+        double gv[12];                                                                            \
+        qx_epi_deri<NA, NB, NC, ND, CMASK, XMASK, DINV, NCOMB, R>(A_, P1, P2, dv, gv);            \
+        if (A_.e.epi == QX_EPI_GRAD) {                                                            \
+          const int at[4] = {A_.pr.atom_a[P1], A_.pr.atom_b[P1], A_.pr.atom_a[P2], A_.pr.atom_b[P2]}; \
+          _Pragma("unroll") for (int k = 0; k < 12; ++k) {                                        \
+            if (!(((CMASK | (DINV ? 8 : 0)) >> (k / 3)) & 1)) continue;                           \
+            const int slot = 3 * at[k / 3] + k % 3;                                               \
+            if (lds) atomicAdd(&sg[slot], gv[k]);                                                 \
+            else qx_atomic_add(&A_.e.grad[slot], gv[k]);                                          \
+          }                                                                                       \
+        }                                                                                         \
+      }                                                                                           \
+    }                                                                                             \
+    if (lds) {                                                                                    \
+      __syncthreads();                                                                            \
+      for (int i = threadIdx.x; i < ng; i += QX_BLOCK)                                            \
+        if (sg[i] != 0.0) qx_atomic_add(&A_.e.grad[i], sg[i]);                                    \
+    }                                                                                             \
   }                                                                                               \
   extern "C" int qx_launch_##NAME(const QxArgs* a, const QxLaunch* L) {                           \
     if (a->w.nquart <= 0) return 0;                                                               \

@@ -4,7 +4,10 @@ Follows quax/methods/hartree_fock.py:8-119: Cholesky orthogonalisation
 (energy_utils.py:29-36), zero (or core) density guess, optional damping of
 the first 10 iterations, optional 1e-9 spectral shift, energy
 E = sum (F + H) D + E_nuc, convergence on |dE| and the RMS of
-A (F D S - S D F) A.  The reference contracts a stored nbf^4 tensor
+A (F D S - S D F) A.  The orbital gradient is evaluated with the density that
+built F; the reference pairs F with the next density, which commutes with F
+by construction, so there only |dE| decides convergence and the density can
+stay ~1e-8 from stationarity -- visible in gradients at the 1e-9 bar.  The reference contracts a stored nbf^4 tensor
 (hartree_fock.py:79-80); here J and K come from the direct J/K kernels, so
 nothing n^4 is ever stored.  DIIS extrapolation (Pulay) is an optional
 addition the reference lacks (it only reports the DIIS error).
@@ -151,14 +154,14 @@ def restricted_hartree_fock(engine, nelectrons, options=None, return_aux_data=Fa
         J, K = build_jk(D)
         F = H + 2 * J - K
         Fu = F
+        # orbital gradient A (F D S - S D F) A of the density D that built F
+        err = A @ (F @ D @ S - S @ D @ F) @ A
         if opts["diis"]:
-            err = A @ (F @ D @ S - S @ D @ F) @ A
             F = diis_extrapolate(F, err)
         E_new, D, C, eps = rhf_iter(F, D_old)
         if opts["diis"]:
             # energy of the density that built the (unextrapolated) Fock matrix
             E_new = torch.sum((Fu + H) * D_old) + Enuc
-        err = A @ (Fu @ D @ S - S @ D @ Fu) @ A
         drms = float(torch.sqrt(torch.mean(err ** 2)))
         dE = float(E - E_new)
         E = E_new

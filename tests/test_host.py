@@ -1,0 +1,138 @@
+"""Host-side checks (no GPU): the index contract, basis data, synthetic
+workloads, the spherical transform, and that the C-ABI library loads and
+exports every entry point include/quax_amd.h declares."""
+import ctypes
+import itertools
+import json
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle
+from quax_amd import utils, workloads
+from quax_amd.basis import BasisSet, cart_overlap_shared_norm, sph_transform
+from quax_amd.molecule import Molecule, read_xyz
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_how_many_derivs():
+    for k in range(1, 6):
+        for n in range(0, 4):
+            assert utils.how_many_derivs(k, n) == oracle.how_many_derivs(k, n) == math.comb(3 * k + n - 1, n)
+
+
+@pytest.mark.parametrize("natom,order", [(1, 1), (2, 1), (3, 1), (2, 2), (3, 2), (2, 3)])
+def test_deriv_vec_index_matches_enumeration(natom, order):
+    dim = 3 * natom
+    for idx, c in enumerate(itertools.combinations_with_replacement(range(dim), order)):
+        dv = np.zeros(dim, dtype=int)
+        for i in c:
+            dv[i] += 1
+        assert utils.get_deriv_vec_idx(dv) == idx == oracle.get_deriv_vec_idx(dv)
+
+
+def test_multi_index_lookup():
+    assert utils.generate_multi_index_lookup(12, 1) == [[i] for i in range(12)]
+    lk = utils.generate_multi_index_lookup(12, 2)
+    assert len(lk) == 78 and lk[0] == [0, 0] and lk[1] == [0, 1] and lk[-1] == [11, 11]
+
+
+def test_native_lookup_and_counts():
+    from quax_amd import _lib
+    L = _lib.load()
+    for n, k in [(9, 1), (12, 2), (6, 3)]:
+        cnt = L.qx_generate_multi_index_lookup(n, k, None, 0)
+        buf = np.zeros(cnt * k, dtype=np.int32)
+        L.qx_generate_multi_index_lookup(n, k, buf.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), cnt)
+        assert buf.reshape(cnt, k).tolist() == utils.generate_multi_index_lookup(n, k)
+        assert L.qx_how_many_derivs(n // 3, k) == utils.how_many_derivs(n // 3, k)
+
+
+def test_library_exports_every_declared_symbol():
+    from quax_amd import _lib
+    L = _lib.load()
+    header = open(os.path.join(ROOT, "include", "quax_amd.h")).read()
+    names = sorted(set(re.findall(r"\b(qx_[a-z0-9_]+)\s*\(", header)))
+    assert len(names) >= 20
+    for nm in names:
+        assert hasattr(L, nm), nm
+        assert nm in _lib.SIGNATURES, nm
+
+
+def test_library_rejects_without_gpu():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("GPU present")
+    from quax_amd._lib import NativeUnavailable
+    from quax_amd.engine import EriEngine
+    mol = workloads.water_sto3g()
+    with pytest.raises(NativeUnavailable):
+        EriEngine(mol, BasisSet.build(mol, "sto-3g"))
+
+
+def test_class_table_static_info():
+    from quax_amd import _lib
+    L = _lib.load()
+    nc = L.qx_num_classes()
+    assert nc == 21
+    seen = set()
+    for k in range(nc):
+        l4 = (ctypes.c_int * 4)()
+        npart = ctypes.c_int()
+        assert L.qx_class_info(k, l4, ctypes.byref(npart)) == 0
+        la, lb, lc, ld = l4[:]
+        assert la >= lb and lc >= ld and (la * (la + 1) // 2 + lb) >= (lc * (lc + 1) // 2 + ld)
+        assert 1 <= npart.value <= 12
+        seen.add((la, lb, lc, ld))
+        ef = ctypes.c_double()
+        pf = (ctypes.c_double * 12)()
+        pc = (ctypes.c_int * 12)()
+        assert L.qx_class_flops(k, ctypes.byref(ef), pf, pc) == 0
+        assert ef.value > 0 and all(pf[i] > 0 for i in range(npart.value))
+    assert len(seen) == 21
+
+
+def test_sph_transform_orthonormal():
+    for l in (2, 3):
+        T = sph_transform(l)
+        S = cart_overlap_shared_norm(l)
+        assert np.abs(T @ S @ T.T - np.eye(2 * l + 1)).max() < 1e-14
+
+
+def test_basis_sizes():
+    m = workloads.cluster10()
+    assert BasisSet.build(m, "even-tempered").nbf() == 300
+    assert BasisSet.build(m, "even-tempered", pure=True).nbf() == 280
+    assert BasisSet.build(workloads.cluster10_subset4(), "even-tempered", pure=True).nbf() == 112
+    assert BasisSet.build(workloads.cluster24(), "even-tempered", pure=True).nbf() == 672
+    w = workloads.water_sto3g()
+    b = BasisSet.build(w, "sto-3g")
+    assert b.nbf() == 7 and [s.l for s in b.shells] == [0, 0, 1, 0, 0]
+
+
+def test_synthetic_geometries_match_fixture():
+    fx = json.load(open(os.path.join(ROOT, "tests", "golden", "geometries.json")))
+    for name, mol in [("cluster10", workloads.cluster10()), ("cluster24", workloads.cluster24())]:
+        assert fx[name]["symbols"] == [mol.symbol(i) for i in range(mol.natom())]
+        assert np.allclose(fx[name]["xyz_bohr"], mol.geometry(), atol=0, rtol=0)
+        g = mol.geometry()
+        d = np.linalg.norm(g[:, None] - g[None], axis=-1) + np.eye(len(g)) * 10
+        assert d.min() >= 1.8 and g.min() >= 0 and g.max() < 8.0
+    b = workloads.batch256()
+    assert len(b) == 256 and len(fx["batch256"]) == 256
+    for m, f in zip(b, fx["batch256"]):
+        assert f["symbols"] == [m.symbol(i) for i in range(m.natom())]
+        assert 3 <= m.natom() <= 12 and m.nelectron() % 2 == 0
+
+
+def test_xyz_round_trip(tmp_path):
+    m = workloads.cluster10()
+    p = str(tmp_path / "g.xyz")
+    m.save_xyz_file(p, True)
+    syms, xyz = read_xyz(p)
+    assert syms == [m.symbol(i) for i in range(m.natom())]
+    assert np.abs(xyz - m.geometry()).max() < 1e-14

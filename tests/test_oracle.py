@@ -1,0 +1,248 @@
+"""Pins for the CPU oracle (no GPU): Boys function against mpmath, closed
+forms, an independent McMurchie-Davidson implementation, textbook H2/STO-3G
+values, symmetry and invariance properties, finite differences, and the
+reference's loop semantics (screening, scatter, single-atom skip)."""
+import math
+
+import mpmath
+import numpy as np
+import pytest
+
+import md_reference as md
+import oracle
+from quax_amd import workloads
+from quax_amd.basis import BasisSet, Shell
+from quax_amd.molecule import Molecule
+
+mpmath.mp.dps = 40
+
+
+def boys_exact(m, T):
+    if T == 0:
+        return 1.0 / (2 * m + 1)
+    h = mpmath.mpf(m) + mpmath.mpf(1) / 2
+    return float(mpmath.gammainc(h, 0, mpmath.mpf(T)) / (2 * mpmath.mpf(T) ** h))
+
+
+@pytest.mark.parametrize("T", [0.0, 1e-12, 1e-6, 0.03, 0.5, 1.7, 5.0, 11.3, 29.999, 30.001, 36.0, 47.5, 120.0, 1e4])
+def test_boys_matches_mpmath(T):
+    F = oracle.boys(16, T)
+    for m in range(17):
+        ref = boys_exact(m, T)
+        assert abs(F[m] - ref) <= 4e-16 * ref, (m, T)
+
+
+def test_boys_identities():
+    for T in (0.3, 4.0, 25.0, 60.0):
+        F = oracle.boys(10, T)
+        for m in range(1, 11):  # downward recursion identity
+            assert abs(F[m - 1] - (2 * T * F[m] + math.exp(-T)) / (2 * m - 1)) < 1e-15 * F[m - 1]
+    F = oracle.boys(5, 0.0)
+    assert np.allclose(F, [1 / (2 * m + 1) for m in range(6)], rtol=0, atol=1e-16)
+    T = 400.0  # large-T asymptote (2m-1)!!/2^(m+1) sqrt(pi/T^(2m+1))
+    F = oracle.boys(6, T)
+    for m in range(7):
+        df = np.prod(np.arange(2 * m - 1, 0, -2)) if m > 0 else 1.0
+        assert abs(F[m] - df / 2 ** (m + 1) * math.sqrt(math.pi / T ** (2 * m + 1))) < 1e-15 * F[m]
+
+
+def _single(sym_xyz, shells):
+    syms = [s for s, _ in sym_xyz]
+    mol = Molecule(syms, [x for _, x in sym_xyz])
+    return mol, BasisSet("t", [Shell(*sh) for sh in shells])
+
+
+def test_ssss_closed_form():
+    rng = np.random.default_rng(1)
+    for _ in range(5):
+        X = rng.uniform(-1.5, 1.5, (4, 3))
+        e = rng.uniform(0.2, 4.0, 4)
+        mol, bs = _single([("H", X[i]) for i in range(4)], [(0, i, [e[i]], [1.0]) for i in range(4)])
+        ob = oracle.OracleBasis(mol, bs)
+        v = ob.shell_quartet(0, 1, 2, 3)[0, 0, 0, 0]
+        a, b, c, d = e
+        p, q = a + b, c + d
+        P = (a * X[0] + b * X[1]) / p
+        Q = (c * X[2] + d * X[3]) / q
+        N = np.prod([(2 * x / math.pi) ** 0.75 for x in e])
+        T = p * q / (p + q) * np.sum((P - Q) ** 2)
+        F0 = 1.0 if T == 0 else 0.5 * math.sqrt(math.pi / T) * math.erf(math.sqrt(T))
+        ref = N * 2 * math.pi ** 2.5 / (p * q * math.sqrt(p + q)) * math.exp(
+            -a * b / p * np.sum((X[0] - X[1]) ** 2) - c * d / q * np.sum((X[2] - X[3]) ** 2)) * F0
+        assert abs(v - ref) < 1e-15 * max(1.0, abs(ref))
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_hgp_matches_mcmurchie_davidson(seed):
+    rng = np.random.default_rng(seed)
+    mol = Molecule(["C", "H", "O", "N"], rng.uniform(-1.5, 1.5, (4, 3)))
+    shells = []
+    for a in range(4):
+        for l in range(3):
+            k = int(rng.integers(1, 3))
+            shells.append(Shell(l, a, list(rng.uniform(0.3, 3, k)), list(rng.uniform(0.2, 1, k))))
+    bs = BasisSet("rand", shells)
+    ob = oracle.OracleBasis(mol, bs)
+    for _ in range(6):
+        q = rng.integers(0, len(shells), 4)
+        blk = ob.shell_quartet(*q)
+        ref = md.eri_contracted([(shells[s].l, shells[s].exps, shells[s].coefs, mol.geometry()[shells[s].atom])
+                                 for s in q])
+        assert np.abs(blk - ref).max() < 1e-14 * max(1.0, np.abs(ref).max())
+
+
+def test_h2_sto3g_textbook():
+    """Szabo & Ostlund, Modern Quantum Chemistry, ch. 3 (R = 1.4 bohr)."""
+    mol = Molecule(["H", "H"], [[0, 0, 0], [0, 0, 1.4]])
+    ob = oracle.OracleBasis(mol, BasisSet.build(mol, "sto-3g"))
+    G = ob.compute_2e_int(1e-14)
+    assert abs(G[0, 0, 0, 0] - 0.7746) < 1e-4
+    assert abs(G[0, 0, 1, 1] - 0.5697) < 1e-4
+    assert abs(G[1, 0, 0, 0] - 0.4441) < 1e-4
+    assert abs(G[1, 0, 1, 0] - 0.2970) < 1e-4
+    S, T, V = ob.oei()
+    assert abs(S[0, 1] - 0.6593) < 1e-4
+    assert abs(T[0, 0] - 0.7600) < 1e-4 and abs(T[0, 1] - 0.2365) < 1e-4
+    assert abs(V[0, 0] - (-1.8804)) < 1e-4 and abs(V[0, 1] - (-1.1948)) < 1e-4
+    scf = oracle.rhf(S, T, V, G, 2, 1.0 / 1.4)
+    assert abs(scf["E"] - (-1.1167)) < 1e-4
+
+
+def test_normalisation_convention():
+    """Libint convention: x^l components have unit norm, xy of a d shell 1/3."""
+    mol = Molecule(["O"], [[0.1, -0.2, 0.3]])
+    bs = BasisSet("t", [Shell(2, 0, [1.3, 0.4], [0.6, 0.5]), Shell(1, 0, [0.9], [1.0])])
+    S, _, _ = oracle.OracleBasis(mol, bs).oei()
+    assert abs(S[0, 0] - 1) < 1e-14 and abs(S[3, 3] - 1) < 1e-14 and abs(S[5, 5] - 1) < 1e-14
+    assert abs(S[1, 1] - 1 / 3) < 1e-14 and abs(S[2, 2] - 1 / 3) < 1e-14 and abs(S[4, 4] - 1 / 3) < 1e-14
+    assert abs(S[0, 3] - 1 / 3) < 1e-14
+    assert np.allclose(np.diag(S)[6:], 1.0, atol=1e-14)
+
+
+def test_eight_fold_symmetry_and_psd():
+    mol = workloads.water_sto3g()
+    G = oracle.OracleBasis(mol, BasisSet.build(mol, "sto-3g")).compute_2e_int(1e-14)
+    for perm in [(1, 0, 2, 3), (0, 1, 3, 2), (2, 3, 0, 1), (3, 2, 1, 0)]:
+        assert np.abs(G - G.transpose(perm)).max() < 1e-14
+    n = G.shape[0]
+    assert np.linalg.eigvalsh(G.reshape(n * n, n * n)).min() > -1e-12
+
+
+def _cluster3():
+    rng = np.random.default_rng(7)
+    mol = Molecule(["C", "H", "O"], rng.uniform(0, 3.0, (3, 3)))
+    shells = []
+    for a in range(3):
+        shells += [Shell(0, a, [0.3], [1.0]), Shell(1, a, [0.45, 1.4], [0.5, 0.6]), Shell(2, a, [0.4], [1.0])]
+    return mol, BasisSet("t", shells)
+
+
+def test_derivative_translational_invariance_and_fd():
+    mol, bs = _cluster3()
+    ob = oracle.OracleBasis(mol, bs)
+    q = (2, 4, 7, 5)   # d p | s p on atoms 0, 1, 2, 1
+    d = ob.shell_quartet_deriv1(*q)
+    inv = d[0:3] + d[3:6] + d[6:9] + d[9:12]
+    assert np.abs(inv).max() < 1e-13 * max(1, np.abs(d).max())
+    h = 1e-4
+    geom = mol.geometry()
+    for atom, x in [(0, 0), (1, 2), (2, 1)]:
+        gp, gm = geom.copy(), geom.copy()
+        gp[atom, x] += h
+        gm[atom, x] -= h
+        fp = oracle.OracleBasis(Molecule(["C", "H", "O"], gp), bs).shell_quartet(*q)
+        fm = oracle.OracleBasis(Molecule(["C", "H", "O"], gm), bs).shell_quartet(*q)
+        fd = (fp - fm) / (2 * h)
+        an = sum(d[3 * c + x] for c in range(4) if bs.shells[q[c]].atom == atom)
+        assert np.abs(fd - an).max() < 1e-7
+
+
+def test_eri_deriv_core_semantics():
+    """eri_deriv_core(1) slices equal compute_2e_deriv for one-hot vectors,
+    single-atom quartets are skipped, and the tensor is the finite
+    difference of compute_2e_int."""
+    mol, bs = _cluster3()
+    ob = oracle.OracleBasis(mol, bs)
+    dG = ob.eri_deriv_core1(1e-14)
+    for k in (0, 4, 8):
+        dv = np.zeros(9, dtype=int)
+        dv[k] = 1
+        assert np.abs(ob.compute_2e_deriv1(1e-14, dv) - dG[k]).max() < 1e-15
+    # a quartet entirely on atom 0 is exactly zero in every slice
+    f0 = [i for i, s in enumerate(bs.shells) if s.atom == 0]
+    bf = ob.bf
+    i0 = bf[f0[0]]
+    assert np.all(dG[:, i0, i0, i0, i0] == 0.0)
+    h = 1e-4
+    geom = mol.geometry()
+    gp, gm = geom.copy(), geom.copy()
+    gp[1, 2] += h
+    gm[1, 2] -= h
+    Gp = oracle.OracleBasis(Molecule(["C", "H", "O"], gp), bs).compute_2e_int(1e-14)
+    Gm = oracle.OracleBasis(Molecule(["C", "H", "O"], gm), bs).compute_2e_int(1e-14)
+    fd = (Gp - Gm) / (2 * h)
+    mask = np.ones_like(fd, dtype=bool)
+    mask[np.ix_(*[np.array([j for j in range(ob.nbf) if bs.shells[np.searchsorted(bf, j, 'right') - 1].atom == 1])] * 4)] = False
+    assert np.abs((fd - dG[5])[mask]).max() < 1e-7
+
+
+def test_screening_semantics():
+    mol, bs = _cluster3()
+    ob = oracle.OracleBasis(mol, bs)
+    vals, pairs = ob.schwarz(1e-3)
+    mx = vals.max()
+    for s1 in range(len(bs.shells)):
+        for s2 in range(s1 + 1):
+            kept = any((p == (s1, s2)).all() for p in pairs)
+            assert kept == (vals[s1, s2] >= 1e-6 / mx)
+    # Schwarz values are diagonal maxima of (ab|ab) (not square-rooted)
+    a, b = pairs[3]
+    blk = ob.shell_quartet(a, b, a, b)
+    na, nb = blk.shape[0], blk.shape[1]
+    assert abs(vals[a, b] - max(abs(blk[i, j, i, j]) for i in range(na) for j in range(nb))) < 1e-16
+
+
+def test_oei_derivatives_fd():
+    mol, bs = _cluster3()
+    ob = oracle.OracleBasis(mol, bs)
+    dS, dT, dV = ob.oei_deriv1()
+    for A in range(3):
+        s = sum(dS[3 * A:3 * A + 3] for A in range(3))
+    assert np.abs(sum(dS[3 * A + 0] for A in range(3))).max() < 1e-13
+    assert np.abs(sum(dV[3 * A + 1] for A in range(3))).max() < 1e-12
+    h = 1e-4
+    geom = mol.geometry()
+    for atom, x in [(0, 1), (2, 2)]:
+        gp, gm = geom.copy(), geom.copy()
+        gp[atom, x] += h
+        gm[atom, x] -= h
+        p = oracle.OracleBasis(Molecule(["C", "H", "O"], gp), bs).oei()
+        m = oracle.OracleBasis(Molecule(["C", "H", "O"], gm), bs).oei()
+        for k, d in enumerate((dS, dT, dV)):
+            assert np.abs((p[k] - m[k]) / (2 * h) - d[3 * atom + x]).max() < 2e-7
+
+
+def test_rhf_gradient_fd_and_invariance():
+    g = np.load(__import__("os").path.join(__import__("os").path.dirname(__file__), "golden", "water_sto3g.npz"))
+    grad = g["gradient"]
+    assert np.abs(grad.reshape(3, 3).sum(0)).max() < 1e-10
+    mol = workloads.water_sto3g()
+
+    def energy(xyz):
+        m = Molecule(["O", "H", "H"], xyz)
+        ob = oracle.OracleBasis(m, BasisSet.build(m, "sto-3g"))
+        S, T, V = ob.oei()
+        Z = np.array([8.0, 1, 1])
+        return oracle.rhf(S, T, V, ob.compute_2e_int(1e-14), 10, oracle.nuclear_repulsion(xyz, Z), conv=1e-14,
+                          maxit=500)["E"]
+
+    x0 = mol.geometry().reshape(-1)
+    for k in (1, 5):
+        vals = []
+        for h in (1e-3, 5e-4):
+            xp, xm = x0.copy(), x0.copy()
+            xp[k] += h
+            xm[k] -= h
+            vals.append((energy(xp.reshape(3, 3)) - energy(xm.reshape(3, 3))) / (2 * h))
+        rich = (4 * vals[1] - vals[0]) / 3
+        assert abs(rich - grad[k]) < 3e-8
