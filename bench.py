@@ -60,10 +60,10 @@ def kernel_table(eng):
         N = ncart(la) * ncart(lb) * ncart(lc) * ncart(ld)
         nm = f"{la}{lb}{lc}{ld}"
         # J/K epilogue: 6 fused multiply-adds per integral
-        info[k] = (f"qx_k_e{nm}", ef.value + 12.0 * N)
+        info[k] = (f"qx_kj_e{nm}", ef.value + 12.0 * N)
         for p in range(npart.value):
             # adjoint epilogue: density pair product (8 flops) + 2 per block
-            info[nc + 12 * k + p] = (f"qx_k_d{nm}_{p}", pf[p] + 8.0 * N + 2.0 * pc[p] * N)
+            info[nc + 12 * k + p] = (f"qx_kg_d{nm}_{p}", pf[p] + 8.0 * N + 2.0 * pc[p] * N)
     return info
 
 

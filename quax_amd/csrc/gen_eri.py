@@ -146,16 +146,51 @@ class HRR:
 
 
 def build_dag(targets):
+    """HRR + VRR DAG for the targets; also returns, per target, the number of
+    HRR lines after which its value is available (emission is in target
+    order, so consumers can be interleaved to keep live ranges short)."""
     best = None
     for order in "bd":
         h = HRR(order)
-        outs = [h.get(*t) for t in targets]
+        outs, pos = [], []
+        for t in targets:
+            outs.append(h.get(*t))
+            pos.append(len(h.lines))
         v = VRR()
         vn = {ac: v.get(ac[0], ac[1], 0) for ac in sorted(h.need)}
         cost = len(h.lines) + len(v.lines)
         if best is None or cost < best[0]:
-            best = (cost, h, v, vn, outs)
+            best = (cost, h, v, vn, outs, pos)
     return best[1:]
+
+
+def interleave(hlines, groups):
+    """groups: list of (position, [statements]) in non-decreasing position;
+    returns HRR lines with each group's statements placed as soon as its
+    inputs exist."""
+    out = []
+    gi = 0
+    for i in range(len(hlines) + 1):
+        while gi < len(groups) and groups[gi][0] <= i:
+            out += groups[gi][1]
+            gi += 1
+        if i < len(hlines):
+            out.append(hlines[i])
+    return out
+
+
+def emit_dblocks(L, nn):
+    """Loads the six density blocks of the quartet (double) from A_.e.D."""
+    NA, NB, NC, ND = nn
+    L.append("  const int n_ = A_.e.nbf;")
+    L.append("  const double* Dm_ = A_.e.D;")
+    L.append("  const int ia_ = A_.pr.bfa[P1], ib_ = A_.pr.bfb[P1], ic_ = A_.pr.bfa[P2], id_ = A_.pr.bfb[P2];")
+    for nm, r0, nr, c0, nc in (("Dab", "ia_", NA, "ib_", NB), ("Dcd", "ic_", NC, "id_", ND),
+                               ("Dac", "ia_", NA, "ic_", NC), ("Dad", "ia_", NA, "id_", ND),
+                               ("Dbc", "ib_", NB, "ic_", NC), ("Dbd", "ib_", NB, "id_", ND)):
+        L.append(f"  double {nm}[{nr * nc}];")
+        L.append(f"  for (int r_ = 0; r_ < {nr}; ++r_) for (int c_ = 0; c_ < {nc}; ++c_) "
+                 f"{nm}[r_ * {nc} + c_] = Dm_[({r0} + r_) * n_ + {c0} + c_];")
 
 
 CENTRES = "ABCD"
@@ -186,13 +221,13 @@ def deriv_targets(l, part):
 def deriv_parts(l):
     """Choose how to split the derivative of class l into kernels."""
     full = [(c, x) for c in "ABC" for x in range(3)]
-    h, v, _, _ = build_dag(deriv_targets(l, full))
+    h, v, _, _, _ = build_dag(deriv_targets(l, full))
     if len(h.lines) + len(v.lines) <= SPLIT_NODES:
         return [("ABC", "xyz", True, full)]
     parts = []
     for c in CENTRES:
         pc = [(c, x) for x in range(3)]
-        h, v, _, _ = build_dag(deriv_targets(l, pc))
+        h, v, _, _, _ = build_dag(deriv_targets(l, pc))
         if len(h.lines) + len(v.lines) <= SPLIT_NODES:
             parts.append((c, "xyz", False, pc))
         else:
@@ -245,71 +280,133 @@ def emit_vrr_consts(L, vrr_lines):
         L.append("  const R rq = (R)(p * opq);")
 
 
-def emit_eri_core(l):
-    la, lb, lc, ld = l
-    N = ncart(la) * ncart(lb) * ncart(lc) * ncart(ld)
-    T = [(a, b, c, d) for a in comps(la) for b in comps(lb) for c in comps(lc) for d in comps(ld)]
-    h, v, vn, outs = build_dag(T)
-    Lt = sum(l)
-    nm = "".join(map(str, l))
-    L = [f"template <typename R>\n__device__ __forceinline__ void qx_core_e{nm}(const QxArgs& A_, int P1, int P2, "
-         f"int u1, int u2, R (&o)[{N}]) {{"]
+def _head(L, fn, sig, Lt, v, vn):
+    L.append(f"template <typename R>\n__device__ __forceinline__ void {fn}(const QxArgs& A_, int P1, int P2, "
+             f"int u1, int u2{sig}) {{")
     emit_setup(L, Lt, False)
     emit_vrr_consts(L, v.lines)
     L += ["  " + x for x in v.lines]
     for (a, c), name in sorted(vn.items()):
         L.append(f"  const R s_{a[0]}{a[1]}{a[2]}_{c[0]}{c[1]}{c[2]} = {name};")
-    L += ["  " + x for x in h.lines]
-    for k, o in enumerate(outs):
-        L.append(f"  o[{k}] = {o};")
+
+
+def emit_eri_core(l):
+    """Two bodies per class: qx_core_e* writes the integrals to an array (the
+    store / tensor-scatter / Schwarz epilogues), qx_jk_e* contracts each
+    integral into the J' and K' accumulators as soon as it is formed."""
+    la, lb, lc, ld = l
+    nn = (ncart(la), ncart(lb), ncart(lc), ncart(ld))
+    NA, NB, NC, ND = nn
+    N = NA * NB * NC * ND
+    T = [(a, b, c, d) for a in comps(la) for b in comps(lb) for c in comps(lc) for d in comps(ld)]
+    h, v, vn, outs, pos = build_dag(T)
+    Lt = sum(l)
+    nm = "".join(map(str, l))
+    # array form
+    L = []
+    _head(L, f"qx_core_e{nm}", f", R (&o)[{N}]", Lt, v, vn)
+    groups = [(pos[k], [f"  o[{k}] = {outs[k]};"]) for k in range(N)]
+    L += ["  " + x if not x.startswith("  ") else x for x in interleave(h.lines, groups)]
     L.append("}")
+    # streaming J/K form
+    J = []
+    _head(J, f"qx_jk_e{nm}", ", double (&jab)[%d], double (&jcd)[%d], double (&kac)[%d], double (&kad)[%d], "
+          "double (&kbc)[%d], double (&kbd)[%d]" % (NA * NB, NC * ND, NA * NC, NA * ND, NB * NC, NB * ND), Lt, v, vn)
+    emit_dblocks(J, nn)
+    groups = []
+    k = 0
+    for i in range(NA):
+        for j in range(NB):
+            for kk in range(NC):
+                for ll in range(ND):
+                    ij, kl, ik, il, jk, jl = i * NB + j, kk * ND + ll, i * NC + kk, i * ND + ll, j * NC + kk, j * ND + ll
+                    groups.append((pos[k], [
+                        f"  {{ const double v_ = (double){outs[k]};",
+                        f"    jab[{ij}] += v_ * Dcd[{kl}]; jcd[{kl}] += v_ * Dab[{ij}];",
+                        f"    kac[{ik}] += v_ * Dbd[{jl}]; kad[{il}] += v_ * Dbc[{jk}];",
+                        f"    kbc[{jk}] += v_ * Dad[{il}]; kbd[{jl}] += v_ * Dac[{ik}]; }}"]))
+                    k += 1
+    J += ["  " + x if not x.startswith("  ") else x for x in interleave(h.lines, groups)]
+    J.append("}")
     stats = dict(vrr=len(v.lines), vrr_flops=v.nflop, hrr=len(h.lines), hrr_flops=h.nflop, L=Lt, N=N,
                  flops=core_flops(Lt, v.nflop, h.nflop))
-    return "\n".join(L), stats
+    return "\n".join(L), "\n".join(J), stats
 
 
 def emit_deriv_core(l, pi, part):
+    """Two bodies per derivative part: qx_core_d* writes every derivative
+    block to an array (tensor scatter), qx_gr_d* contracts each derivative
+    integral with the density pair product as soon as it is formed (the
+    adjoint gradient), so no derivative block is ever held whole."""
     la, lb, lc, ld = l
-    N = ncart(la) * ncart(lb) * ncart(lc) * ncart(ld)
+    nn = (ncart(la), ncart(lb), ncart(lc), ncart(ld))
+    NA, NB, NC_, ND = nn
+    N = NA * NB * NC_ * ND
     cen, xs, dinv, combos = part
     T = deriv_targets(l, combos)
-    h, v, vn, outs = build_dag(T)
+    h, v, vn, outs, pos = build_dag(T)
     Lt = sum(l) + 1
     nm = "".join(map(str, l))
     NC = len(combos)
-    L = [f"template <typename R>\n__device__ __forceinline__ void qx_core_d{nm}_{pi}(const QxArgs& A_, int P1, int P2, "
-         f"int u1, int u2, R (&dv)[{NC * N}]) {{"]
-    emit_setup(L, Lt, True)
-    emit_vrr_consts(L, v.lines)
-    L += ["  " + x for x in v.lines]
-    for (a, c), name in sorted(vn.items()):
-        L.append(f"  const R s_{a[0]}{a[1]}{a[2]}_{c[0]}{c[1]}{c[2]} = {name};")
-    L += ["  " + x for x in h.lines]
     wexpr = {"A": "(R)A_.pp.a2[u1]", "B": "(R)A_.pp.b2[u1]", "C": "(R)A_.pp.a2[u2]", "D": "(R)A_.pp.b2[u2]"}
-    for c in sorted(set(cc for cc, _ in combos)):
-        L.append(f"  const R w{c} = {wexpr[c]};")
-    it = iter(outs)
+    wdecl = [f"  const R w{c} = {wexpr[c]};" for c in sorted(set(cc for cc, _ in combos))]
+    # per element: expressions of each combo and the position after which all exist
+    elems = []
+    ti = 0
     e = 0
     for a in comps(la):
         for b in comps(lb):
             for c in comps(lc):
                 for d in comps(ld):
                     q = [a, b, c, d]
+                    exprs = []
+                    last = 0
                     for ci, (cn, x) in enumerate(combos):
                         k = CENTRES.index(cn)
-                        up = next(it)
+                        up = outs[ti]
+                        last = max(last, pos[ti])
+                        ti += 1
                         n = q[k][x]
-                        dn = next(it) if n > 0 else None
-                        expr = f"w{cn} * {up}" + (f" - R({n}) * {dn}" if dn else "")
-                        L.append(f"  dv[{ci * N + e}] = {expr};")
+                        dn = None
+                        if n > 0:
+                            dn = outs[ti]
+                            last = max(last, pos[ti])
+                            ti += 1
+                        exprs.append(f"w{cn} * {up}" + (f" - R({n}) * {dn}" if dn else ""))
+                    ijkl = ((["abcd".index(ch) for ch in "abcd"]))
+                    elems.append((e, last, exprs, (a, b, c, d)))
                     e += 1
+    # array form
+    L = []
+    _head(L, f"qx_core_d{nm}_{pi}", f", R (&dv)[{NC * N}]", Lt, v, vn)
+    L += wdecl
+    groups = [(last, [f"  dv[{ci * N + e}] = {ex};" for ci, ex in enumerate(exprs)]) for (e, last, exprs, _) in elems]
+    L += ["  " + x if not x.startswith("  ") else x for x in interleave(h.lines, groups)]
     L.append("}")
+    # streaming adjoint form
+    G = []
+    _head(G, f"qx_gr_d{nm}_{pi}", f", double (&g)[{NC}]", Lt, v, vn)
+    G += wdecl
+    emit_dblocks(G, nn)
+    groups = []
+    for (e, last, exprs, _) in elems:
+        i, rem = divmod(e, NB * NC_ * ND)
+        j, rem = divmod(rem, NC_ * ND)
+        kk, ll = divmod(rem, ND)
+        ij, kl, ik, il, jk, jl = i * NB + j, kk * ND + ll, i * NC_ + kk, i * ND + ll, j * NC_ + kk, j * ND + ll
+        st = [f"  {{ const double G_ = 2.0 * Dab[{ij}] * Dcd[{kl}] - 0.5 * (Dac[{ik}] * Dbd[{jl}] + "
+              f"Dad[{il}] * Dbc[{jk}]);"]
+        st += [f"    g[{ci}] += G_ * (double)({ex});" for ci, ex in enumerate(exprs)]
+        st.append("  }")
+        groups.append((last, st))
+    G += ["  " + x if not x.startswith("  ") else x for x in interleave(h.lines, groups)]
+    G.append("}")
     cmask = sum(1 << CENTRES.index(c) for c in set(cc for cc, _ in combos))
     xmask = sum(1 << x for x in set(x for _, x in combos))
     stats = dict(part=pi, centres=cen, xs=xs, dinv=dinv, vrr=len(v.lines), vrr_flops=v.nflop,
                  hrr=len(h.lines), hrr_flops=h.nflop, L=Lt, N=N, ncomb=NC,
                  flops=core_flops(Lt, v.nflop, h.nflop) + 3 * NC * N)
-    return "\n".join(L), stats, (cmask, xmask, dinv, NC)
+    return "\n".join(L), "\n".join(G), stats, (cmask, xmask, dinv, NC)
 
 
 def core_flops(L, vrr_flops, hrr_flops):
@@ -331,30 +428,31 @@ def canonical_classes(lmax):
 
 
 def emit_class_files(cls, outdir, fp32):
-    """Writes cls_<l>_e.hip and one cls_<l>_d<k>.hip per derivative part."""
+    """Writes, per class: cls_<l>_e.hip (array-form ERI kernel),
+    cls_<l>_j.hip (streaming J/K kernel) and, per derivative part,
+    cls_<l>_d<k>.hip (array form) and cls_<l>_g<k>.hip (streaming adjoint)."""
     la, lb, lc, ld = cls
     nm = "".join(map(str, cls))
     dims = f"{ncart(la)}, {ncart(lb)}, {ncart(lc)}, {ncart(ld)}"
     N = ncart(la) * ncart(lb) * ncart(lc) * ncart(ld)
     files, stats = [], []
-    src, st = emit_eri_core(cls)
-    body = ['#include "../qx_device.h"', "", src, "",
-            f"QX_DEFINE_ERI_KERNEL(e{nm}, {N}, {dims}, {1 if fp32 else 0})", ""]
-    path = os.path.join(outdir, f"cls_{nm}_e.hip")
-    with open(path, "w") as fh:
-        fh.write("\n".join(body))
-    files.append(path)
+
+    def write(name, body):
+        path = os.path.join(outdir, name)
+        with open(path, "w") as fh:
+            fh.write("\n".join(['#include "../qx_device.h"', ""] + body + [""]))
+        files.append(path)
+
+    src_e, src_j, st = emit_eri_core(cls)
+    write(f"cls_{nm}_e.hip", [src_e, "", f"QX_DEFINE_ERI_KERNEL(e{nm}, {N}, {dims})"])
+    write(f"cls_{nm}_j.hip", [src_j, "", f"QX_DEFINE_JK_KERNEL(e{nm}, {dims}, {1 if fp32 else 0})"])
     stats.append((cls, "eri", st))
     parts = deriv_parts(cls)
     for pi, part in enumerate(parts):
-        src, st, (cmask, xmask, dinv, NC) = emit_deriv_core(cls, pi, part)
-        body = ['#include "../qx_device.h"', "", src, "",
-                f"QX_DEFINE_DERI_KERNEL(d{nm}_{pi}, {N}, {dims}, {cmask}, {xmask}, {1 if dinv else 0}, {NC}, "
-                f"{1 if fp32 else 0})", ""]
-        path = os.path.join(outdir, f"cls_{nm}_d{pi}.hip")
-        with open(path, "w") as fh:
-            fh.write("\n".join(body))
-        files.append(path)
+        src_d, src_g, st, (cmask, xmask, dinv, NC) = emit_deriv_core(cls, pi, part)
+        args = f"{N}, {dims}, {cmask}, {xmask}, {1 if dinv else 0}, {NC}"
+        write(f"cls_{nm}_d{pi}.hip", [src_d, "", f"QX_DEFINE_DERI_KERNEL(d{nm}_{pi}, {args})"])
+        write(f"cls_{nm}_g{pi}.hip", [src_g, "", f"QX_DEFINE_GRAD_KERNEL(d{nm}_{pi}, {args}, {1 if fp32 else 0})"])
         stats.append((cls, f"deriv{pi}", st))
     return files, stats, len(parts)
 
@@ -378,18 +476,21 @@ def main():
     for cls in classes:
         nm = "".join(map(str, cls))
         D.append(f'extern "C" int qx_launch_e{nm}(const QxArgs*, const QxLaunch*);')
+        D.append(f'extern "C" int qx_launchj_e{nm}(const QxArgs*, const QxLaunch*);')
         for pi in range(nparts[cls]):
             D.append(f'extern "C" int qx_launch_d{nm}_{pi}(const QxArgs*, const QxLaunch*);')
+            D.append(f'extern "C" int qx_launchg_d{nm}_{pi}(const QxArgs*, const QxLaunch*);')
     D.append("")
     D.append("static const QxClassEntry qx_class_table[QX_NUM_CLASSES] = {")
     for cls in classes:
         nm = "".join(map(str, cls))
         dl = ", ".join(f"qx_launch_d{nm}_{pi}" for pi in range(nparts[cls]))
+        gl = ", ".join(f"qx_launchg_d{nm}_{pi}" for pi in range(nparts[cls]))
         ef, pf, pc = flops[cls]
         pfs = ", ".join(f"{x:.1f}" for x in pf)
         pcs = ", ".join(str(x) for x in pc)
-        D.append(f"  {{{cls[0]}, {cls[1]}, {cls[2]}, {cls[3]}, qx_launch_e{nm}, {nparts[cls]}, {{{dl}}}, "
-                 f"{ef:.1f}, {{{pfs}}}, {{{pcs}}}}},")
+        D.append(f"  {{{cls[0]}, {cls[1]}, {cls[2]}, {cls[3]}, qx_launch_e{nm}, qx_launchj_e{nm}, {nparts[cls]}, "
+                 f"{{{dl}}}, {{{gl}}}, {ef:.1f}, {{{pfs}}}, {{{pcs}}}}},")
     D.append("};")
     with open(os.path.join(outdir, "qx_classes.inc"), "w") as fh:
         fh.write("\n".join(D) + "\n")

@@ -196,110 +196,76 @@ __device__ __forceinline__ void qx_scatter8(double* out, size_t n, size_t I, siz
   }
 }
 
-// ---- ERI epilogues ----------------------------------------------------
-template <int NA, int NB, int NC, int ND, typename R>
-__device__ __forceinline__ void qx_epi_jk(const QxArgs& A_, int P1, int P2, const R (&o)[NA * NB * NC * ND],
-                                          bool active) {
-  // J'_ab += 4f (ab|cd) D_cd, J'_cd += 4f (ab|cd) D_ab,
-  // K'_ac += 2f (ab|cd) D_bd, K'_ad += 2f D_bc, K'_bc += 2f D_ad, K'_bd += 2f D_ac.
-  // Lanes of one wavefront usually share the bra pair (consecutive kets of
-  // one bra row): then J'_ab is summed across the wavefront and added once.
-  const bool uni = qx_wave_uniform(P1, active);
-  if (!active) {
-    if (uni) {
-#pragma unroll
-      for (int k = 0; k < NA * NB; ++k) (void)qx_wave_sum(0.0);
-    }
-    return;
-  }
-  const int n = A_.e.nbf;
-  const int ia = A_.pr.bfa[P1], ib = A_.pr.bfb[P1], ic = A_.pr.bfa[P2], id = A_.pr.bfb[P2];
-  const double* D = A_.e.D;
-  double Dab[NA * NB], Dcd[NC * ND], Dac[NA * NC], Dad[NA * ND], Dbc[NB * NC], Dbd[NB * ND];
-#pragma unroll
-  for (int i = 0; i < NA; ++i) {
-#pragma unroll
-    for (int j = 0; j < NB; ++j) Dab[i * NB + j] = D[(ia + i) * n + ib + j];
-#pragma unroll
-    for (int k = 0; k < NC; ++k) Dac[i * NC + k] = D[(ia + i) * n + ic + k];
-#pragma unroll
-    for (int l = 0; l < ND; ++l) Dad[i * ND + l] = D[(ia + i) * n + id + l];
-  }
-#pragma unroll
-  for (int j = 0; j < NB; ++j) {
-#pragma unroll
-    for (int k = 0; k < NC; ++k) Dbc[j * NC + k] = D[(ib + j) * n + ic + k];
-#pragma unroll
-    for (int l = 0; l < ND; ++l) Dbd[j * ND + l] = D[(ib + j) * n + id + l];
-  }
-#pragma unroll
-  for (int k = 0; k < NC; ++k)
-#pragma unroll
-    for (int l = 0; l < ND; ++l) Dcd[k * ND + l] = D[(ic + k) * n + id + l];
-  double jab[NA * NB] = {}, jcd[NC * ND] = {}, kac[NA * NC] = {}, kad[NA * ND] = {}, kbc[NB * NC] = {},
-         kbd[NB * ND] = {};
-#pragma unroll
-  for (int i = 0; i < NA; ++i)
-#pragma unroll
-    for (int j = 0; j < NB; ++j)
-#pragma unroll
-      for (int k = 0; k < NC; ++k)
-#pragma unroll
-        for (int l = 0; l < ND; ++l) {
-          const double v = (double)o[((i * NB + j) * NC + k) * ND + l];
-          jab[i * NB + j] += v * Dcd[k * ND + l];
-          jcd[k * ND + l] += v * Dab[i * NB + j];
-          kac[i * NC + k] += v * Dbd[j * ND + l];
-          kad[i * ND + l] += v * Dbc[j * NC + k];
-          kbc[j * NC + k] += v * Dad[i * ND + l];
-          kbd[j * ND + l] += v * Dac[i * NC + k];
-        }
-  const double f = qx_degeneracy(A_, P1, P2);
+// ---- streaming-kernel epilogues -----------------------------------------
+// Flushes per-thread J'/K' partials: J'_ab += 4f sum_cd (ab|cd) D_cd etc.
+// Every lane calls this (converged); inactive lanes hold zeros.
+template <int NA, int NB, int NC, int ND>
+__device__ __forceinline__ void qx_jk_flush(const QxArgs& A_, int P1, int P2, bool active,
+                                            const double (&jab)[NA * NB], const double (&jcd)[NC * ND],
+                                            const double (&kac)[NA * NC], const double (&kad)[NA * ND],
+                                            const double (&kbc)[NB * NC], const double (&kbd)[NB * ND]) {
+  const bool uni = qx_wave_uniform(P1, active);  // wave-uniform value
+  const bool lead = qx_wave_leader(active);
+  const double f = active ? qx_degeneracy(A_, P1, P2) : 0.0;
   const double fj = 4.0 * f, fk = 2.0 * f;
+  const int n = A_.e.nbf;
   double* J = A_.e.J;
   double* K = A_.e.K;
+  int ia = 0, ib = 0, ic = 0, id = 0;
+  if (active) {
+    ia = A_.pr.bfa[P1]; ib = A_.pr.bfb[P1]; ic = A_.pr.bfa[P2]; id = A_.pr.bfb[P2];
+  }
   if (uni) {
-    const bool lead = qx_wave_leader(true);
 #pragma unroll
     for (int k = 0; k < NA * NB; ++k) {
       const double v = qx_wave_sum(fj * jab[k]);
       if (lead) qx_atomic_add(&J[(ia + k / NB) * n + ib + k % NB], v);
     }
-  } else {
+  } else if (active) {
 #pragma unroll
-    for (int i = 0; i < NA; ++i)
-#pragma unroll
-      for (int j = 0; j < NB; ++j) qx_atomic_add(&J[(ia + i) * n + ib + j], fj * jab[i * NB + j]);
+    for (int k = 0; k < NA * NB; ++k) qx_atomic_add(&J[(ia + k / NB) * n + ib + k % NB], fj * jab[k]);
   }
+  if (!active) return;
 #pragma unroll
-  for (int k = 0; k < NC; ++k)
+  for (int k = 0; k < NC * ND; ++k) qx_atomic_add(&J[(ic + k / ND) * n + id + k % ND], fj * jcd[k]);
 #pragma unroll
-    for (int l = 0; l < ND; ++l) qx_atomic_add(&J[(ic + k) * n + id + l], fj * jcd[k * ND + l]);
+  for (int k = 0; k < NA * NC; ++k) qx_atomic_add(&K[(ia + k / NC) * n + ic + k % NC], fk * kac[k]);
 #pragma unroll
-  for (int i = 0; i < NA; ++i) {
+  for (int k = 0; k < NA * ND; ++k) qx_atomic_add(&K[(ia + k / ND) * n + id + k % ND], fk * kad[k]);
 #pragma unroll
-    for (int k = 0; k < NC; ++k) qx_atomic_add(&K[(ia + i) * n + ic + k], fk * kac[i * NC + k]);
+  for (int k = 0; k < NB * NC; ++k) qx_atomic_add(&K[(ib + k / NC) * n + ic + k % NC], fk * kbc[k]);
 #pragma unroll
-    for (int l = 0; l < ND; ++l) qx_atomic_add(&K[(ia + i) * n + id + l], fk * kad[i * ND + l]);
+  for (int k = 0; k < NB * ND; ++k) qx_atomic_add(&K[(ib + k / ND) * n + id + k % ND], fk * kbd[k]);
+}
+
+// Spreads NCOMB per-(centre, direction) sums to the 12 centre x direction
+// slots (d/dD from translational invariance when DINV), weighted by w.
+template <int CMASK, int XMASK, int DINV, int NCOMB>
+__device__ __forceinline__ void qx_grad_spread(double w, const double (&g)[NCOMB], double (&gv)[12]) {
+  int ci = 0;
+  double dsum[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int cen = 0; cen < 4; ++cen) {
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+      double v = 0.0;
+      if ((CMASK & (1 << cen)) && (XMASK & (1 << x))) v = w * g[ci++];
+      dsum[x] += v;
+      gv[3 * cen + x] = v;
+    }
   }
+  if (DINV) {
 #pragma unroll
-  for (int j = 0; j < NB; ++j) {
-#pragma unroll
-    for (int k = 0; k < NC; ++k) qx_atomic_add(&K[(ib + j) * n + ic + k], fk * kbc[j * NC + k]);
-#pragma unroll
-    for (int l = 0; l < ND; ++l) qx_atomic_add(&K[(ib + j) * n + id + l], fk * kbd[j * ND + l]);
+    for (int x = 0; x < 3; ++x) gv[9 + x] = -dsum[x];
   }
 }
 
+// ---- ERI epilogues ----------------------------------------------------
 template <int NA, int NB, int NC, int ND, typename R>
 __device__ __forceinline__ void qx_epi_eri(const QxArgs& A_, long long t, int P1, int P2,
                                            const R (&o)[NA * NB * NC * ND], bool active) {
   constexpr int N = NA * NB * NC * ND;
   const int epi = A_.e.epi;
-  if (epi == QX_EPI_JK) {
-    qx_epi_jk<NA, NB, NC, ND, R>(A_, P1, P2, o, active);
-    return;
-  }
   if (!active) return;
   if (epi == QX_EPI_STORE) {
     double* out = A_.e.out + A_.w.store_base + t;
@@ -346,51 +312,7 @@ __device__ __forceinline__ void qx_epi_deri(const QxArgs& A_, int P1, int P2, co
   const int at[4] = {A_.pr.atom_a[P1], A_.pr.atom_b[P1], A_.pr.atom_a[P2], A_.pr.atom_b[P2]};
   const int n = A_.e.nbf;
   const int ia = A_.pr.bfa[P1], ib = A_.pr.bfb[P1], ic = A_.pr.bfa[P2], id = A_.pr.bfb[P2];
-  if (A_.e.epi == QX_EPI_GRAD) {
-    // g_X = sum_ijkl dX(ij|kl) G_ijkl, G_ijkl = 2 D_ij D_kl - (D_ik D_jl + D_il D_jk) / 2,
-    // weighted by 8f (all permutations of a canonical quartet).
-    const double* D = A_.e.D;
-    double g[NCOMB];
-#pragma unroll
-    for (int c = 0; c < NCOMB; ++c) g[c] = 0.0;
-#pragma unroll
-    for (int i = 0; i < NA; ++i)
-#pragma unroll
-      for (int j = 0; j < NB; ++j) {
-        const double dij = D[(ia + i) * n + ib + j];
-#pragma unroll
-        for (int k = 0; k < NC; ++k) {
-          const double dik = D[(ia + i) * n + ic + k], djk = D[(ib + j) * n + ic + k];
-#pragma unroll
-          for (int l = 0; l < ND; ++l) {
-            const double gm = 2.0 * dij * D[(ic + k) * n + id + l] -
-                              0.5 * (dik * D[(ib + j) * n + id + l] + D[(ia + i) * n + id + l] * djk);
-            const int e = ((i * NB + j) * NC + k) * ND + l;
-#pragma unroll
-            for (int c = 0; c < NCOMB; ++c) g[c] += gm * (double)dv[c * N + e];
-          }
-        }
-      }
-    // per (centre, direction) contributions, returned to the kernel which
-    // accumulates them per block
-    const double w = 8.0 * qx_degeneracy(A_, P1, P2);
-    int ci = 0;
-    double dsum[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-    for (int cen = 0; cen < 4; ++cen) {
-#pragma unroll
-      for (int x = 0; x < 3; ++x) {
-        double v = 0.0;
-        if ((CMASK & (1 << cen)) && (XMASK & (1 << x))) v = w * g[ci++];
-        dsum[x] += v;
-        gv[3 * cen + x] = v;
-      }
-    }
-    if (DINV) {
-#pragma unroll
-      for (int x = 0; x < 3; ++x) gv[9 + x] = -dsum[x];
-    }
-  } else {  // QX_EPI_DFULL
+  {  // QX_EPI_DFULL
     const size_t nn = n;
     const size_t n4 = nn * nn * nn * nn;
     const bool sab = A_.pr.sa[P1] == A_.pr.sb[P1], scd = A_.pr.sa[P2] == A_.pr.sb[P2], sp = P1 == P2;
@@ -438,38 +360,82 @@ __device__ __forceinline__ void qx_epi_deri(const QxArgs& A_, int P1, int P2, co
 
 #define QX_GRAD_LDS 768  // block-level gradient accumulators (3 * atoms)
 
-#define QX_DEFINE_ERI_KERNEL(NAME, N, NA, NB, NC, ND, WITH32)                                     \
-  template <typename R>                                                                           \
+// Array-form ERI kernel: store / full-tensor scatter / Schwarz epilogues.
+#define QX_DEFINE_ERI_KERNEL(NAME, N, NA, NB, NC, ND)                                             \
   __global__ __launch_bounds__(QX_BLOCK) void qx_k_##NAME(QxArgs A_) {                            \
+    const long long t = (long long)blockIdx.x * QX_BLOCK + threadIdx.x;                          \
+    if (t >= A_.w.nquart) return;                                                                 \
+    int u1, u2, P1, P2;                                                                           \
+    qx_locate(A_, t, u1, u2, P1, P2);                                                             \
+    double o[N];                                                                                  \
+    qx_core_##NAME<double>(A_, P1, P2, u1, u2, o);                                                \
+    qx_epi_eri<NA, NB, NC, ND, double>(A_, t, P1, P2, o, true);                                   \
+  }                                                                                               \
+  extern "C" int qx_launch_##NAME(const QxArgs* a, const QxLaunch* L) {                           \
+    if (a->w.nquart <= 0) return 0;                                                               \
+    if (L->fp32 || a->e.epi == QX_EPI_JK) return -1;                                              \
+    hipLaunchKernelGGL(qx_k_##NAME, QX_GRID(a->w.nquart), dim3(QX_BLOCK), 0, L->stream, *a);      \
+    return (int)hipGetLastError();                                                                \
+  }
+
+// Streaming J/K kernel: every integral is contracted with the density as
+// soon as it is formed; the per-thread J'/K' partials are flushed with
+// atomics, J'_ab summed across the wavefront when all its lanes share the
+// bra pair.  Cross-lane operations run in converged control flow.
+#define QX_DEFINE_JK_KERNEL(NAME, NA, NB, NC, ND, WITH32)                                         \
+  template <typename R>                                                                           \
+  __global__ __launch_bounds__(QX_BLOCK) void qx_kj_##NAME(QxArgs A_) {                           \
     const long long t = (long long)blockIdx.x * QX_BLOCK + threadIdx.x;                          \
     const bool active = t < A_.w.nquart;                                                          \
     int u1 = 0, u2 = 0, P1 = -1, P2 = -1;                                                         \
     if (active) qx_locate(A_, t, u1, u2, P1, P2);                                                 \
-    R o[N];                                                                                       \
-    if (active) qx_core_##NAME<R>(A_, P1, P2, u1, u2, o);                                         \
-    qx_epi_eri<NA, NB, NC, ND, R>(A_, t, P1, P2, o, active);                                      \
+    double jab[NA * NB] = {}, jcd[NC * ND] = {}, kac[NA * NC] = {}, kad[NA * ND] = {},            \
+           kbc[NB * NC] = {}, kbd[NB * ND] = {};                                                  \
+    if (active) qx_jk_##NAME<R>(A_, P1, P2, u1, u2, jab, jcd, kac, kad, kbc, kbd);                \
+    qx_jk_flush<NA, NB, NC, ND>(A_, P1, P2, active, jab, jcd, kac, kad, kbc, kbd);                \
   }                                                                                               \
-  extern "C" int qx_launch_##NAME(const QxArgs* a, const QxLaunch* L) {                           \
+  extern "C" int qx_launchj_##NAME(const QxArgs* a, const QxLaunch* L) {                          \
     if (a->w.nquart <= 0) return 0;                                                               \
+    if (a->e.epi != QX_EPI_JK) return -1;                                                         \
     if (L->fp32) {                                                                                \
-      if (!(WITH32) || a->e.epi != QX_EPI_JK) return -1;                                          \
+      if (!(WITH32)) return -1;                                                                   \
       if constexpr (WITH32)                                                                       \
-        hipLaunchKernelGGL(qx_k_##NAME<float>, QX_GRID(a->w.nquart), dim3(QX_BLOCK), 0, L->stream, *a); \
+        hipLaunchKernelGGL(qx_kj_##NAME<float>, QX_GRID(a->w.nquart), dim3(QX_BLOCK), 0, L->stream, *a); \
     } else {                                                                                      \
-      hipLaunchKernelGGL(qx_k_##NAME<double>, QX_GRID(a->w.nquart), dim3(QX_BLOCK), 0, L->stream, *a); \
+      hipLaunchKernelGGL(qx_kj_##NAME<double>, QX_GRID(a->w.nquart), dim3(QX_BLOCK), 0, L->stream, *a); \
     }                                                                                             \
     return (int)hipGetLastError();                                                                \
   }
 
-// The gradient epilogue accumulates into LDS (one slot per nuclear
-// coordinate) and flushes once per block: FP64 atomics on a few dozen global
-// addresses from every quartet would serialise.
-#define QX_DEFINE_DERI_KERNEL(NAME, N, NA, NB, NC, ND, CMASK, XMASK, DINV, NCOMB, WITH32)         \
-  template <typename R>                                                                           \
+// Array-form derivative kernel: full derivative-tensor scatter.
+#define QX_DEFINE_DERI_KERNEL(NAME, N, NA, NB, NC, ND, CMASK, XMASK, DINV, NCOMB)                 \
   __global__ __launch_bounds__(QX_BLOCK) void qx_k_##NAME(QxArgs A_) {                            \
+    const long long t = (long long)blockIdx.x * QX_BLOCK + threadIdx.x;                          \
+    if (t >= A_.w.nquart) return;                                                                 \
+    int u1, u2, P1, P2;                                                                           \
+    qx_locate(A_, t, u1, u2, P1, P2);                                                             \
+    if (qx_single_atom(A_, P1, P2)) return;                                                       \
+    double dv[NCOMB * N];                                                                         \
+    qx_core_##NAME<double>(A_, P1, P2, u1, u2, dv);                                               \
+    double gv[12];                                                                                \
+    qx_epi_deri<NA, NB, NC, ND, CMASK, XMASK, DINV, NCOMB, double>(A_, P1, P2, dv, gv);           \
+  }                                                                                               \
+  extern "C" int qx_launch_##NAME(const QxArgs* a, const QxLaunch* L) {                           \
+    if (a->w.nquart <= 0) return 0;                                                               \
+    if (L->fp32 || a->e.epi != QX_EPI_DFULL) return -1;                                           \
+    hipLaunchKernelGGL(qx_k_##NAME, QX_GRID(a->w.nquart), dim3(QX_BLOCK), 0, L->stream, *a);      \
+    return (int)hipGetLastError();                                                                \
+  }
+
+// Streaming adjoint-gradient kernel: each derivative integral is contracted
+// with the density pair product when formed; per-thread sums over the 4
+// centres x 3 directions go to LDS and are flushed once per block.
+#define QX_DEFINE_GRAD_KERNEL(NAME, N, NA, NB, NC, ND, CMASK, XMASK, DINV, NCOMB, WITH32)         \
+  template <typename R>                                                                           \
+  __global__ __launch_bounds__(QX_BLOCK) void qx_kg_##NAME(QxArgs A_) {                           \
     __shared__ double sg[QX_GRAD_LDS];                                                            \
     const int ng = 3 * A_.e.natom;                                                                \
-    const bool lds = A_.e.epi == QX_EPI_GRAD && ng <= QX_GRAD_LDS;                                \
+    const bool lds = ng <= QX_GRAD_LDS;                                                           \
     if (lds) {                                                                                    \
       for (int i = threadIdx.x; i < ng; i += QX_BLOCK) sg[i] = 0.0;                               \
       __syncthreads();                                                                            \
@@ -479,18 +445,17 @@ __device__ __forceinline__ void qx_epi_deri(const QxArgs& A_, int P1, int P2, co
       int u1, u2, P1, P2;                                                                         \
       qx_locate(A_, t, u1, u2, P1, P2);                                                           \
       if (!qx_single_atom(A_, P1, P2)) {                                                          \
-        R dv[NCOMB * N];                                                                          \
-        qx_core_##NAME<R>(A_, P1, P2, u1, u2, dv);                                                \
+        double g[NCOMB];                                                                          \
+        _Pragma("unroll") for (int c = 0; c < NCOMB; ++c) g[c] = 0.0;                             \
+        qx_gr_##NAME<R>(A_, P1, P2, u1, u2, g);                                                   \
         double gv[12];                                                                            \
-        qx_epi_deri<NA, NB, NC, ND, CMASK, XMASK, DINV, NCOMB, R>(A_, P1, P2, dv, gv);            \
-        if (A_.e.epi == QX_EPI_GRAD) {                                                            \
-          const int at[4] = {A_.pr.atom_a[P1], A_.pr.atom_b[P1], A_.pr.atom_a[P2], A_.pr.atom_b[P2]}; \
-          _Pragma("unroll") for (int k = 0; k < 12; ++k) {                                        \
-            if (!(((CMASK | (DINV ? 8 : 0)) >> (k / 3)) & 1)) continue;                           \
-            const int slot = 3 * at[k / 3] + k % 3;                                               \
-            if (lds) atomicAdd(&sg[slot], gv[k]);                                                 \
-            else qx_atomic_add(&A_.e.grad[slot], gv[k]);                                          \
-          }                                                                                       \
+        qx_grad_spread<CMASK, XMASK, DINV, NCOMB>(8.0 * qx_degeneracy(A_, P1, P2), g, gv);        \
+        const int at[4] = {A_.pr.atom_a[P1], A_.pr.atom_b[P1], A_.pr.atom_a[P2], A_.pr.atom_b[P2]}; \
+        _Pragma("unroll") for (int k = 0; k < 12; ++k) {                                          \
+          if (!(((CMASK | (DINV ? 8 : 0)) >> (k / 3)) & 1)) continue;                             \
+          const int slot = 3 * at[k / 3] + k % 3;                                                 \
+          if (lds) atomicAdd(&sg[slot], gv[k]);                                                   \
+          else qx_atomic_add(&A_.e.grad[slot], gv[k]);                                            \
         }                                                                                         \
       }                                                                                           \
     }                                                                                             \
@@ -500,23 +465,26 @@ __device__ __forceinline__ void qx_epi_deri(const QxArgs& A_, int P1, int P2, co
         if (sg[i] != 0.0) qx_atomic_add(&A_.e.grad[i], sg[i]);                                    \
     }                                                                                             \
   }                                                                                               \
-  extern "C" int qx_launch_##NAME(const QxArgs* a, const QxLaunch* L) {                           \
+  extern "C" int qx_launchg_##NAME(const QxArgs* a, const QxLaunch* L) {                          \
     if (a->w.nquart <= 0) return 0;                                                               \
+    if (a->e.epi != QX_EPI_GRAD) return -1;                                                       \
     if (L->fp32) {                                                                                \
-      if (!(WITH32) || a->e.epi != QX_EPI_GRAD) return -1;                                        \
+      if (!(WITH32)) return -1;                                                                   \
       if constexpr (WITH32)                                                                       \
-        hipLaunchKernelGGL(qx_k_##NAME<float>, QX_GRID(a->w.nquart), dim3(QX_BLOCK), 0, L->stream, *a); \
+        hipLaunchKernelGGL(qx_kg_##NAME<float>, QX_GRID(a->w.nquart), dim3(QX_BLOCK), 0, L->stream, *a); \
     } else {                                                                                      \
-      hipLaunchKernelGGL(qx_k_##NAME<double>, QX_GRID(a->w.nquart), dim3(QX_BLOCK), 0, L->stream, *a); \
+      hipLaunchKernelGGL(qx_kg_##NAME<double>, QX_GRID(a->w.nquart), dim3(QX_BLOCK), 0, L->stream, *a); \
     }                                                                                             \
     return (int)hipGetLastError();                                                                \
   }
 
 struct QxClassEntry {
   int la, lb, lc, ld;
-  int (*eri)(const QxArgs*, const QxLaunch*);
+  int (*eri)(const QxArgs*, const QxLaunch*);      // array form (store, scatter, Schwarz)
+  int (*jk)(const QxArgs*, const QxLaunch*);       // streaming J/K
   int nparts;
-  int (*deri[12])(const QxArgs*, const QxLaunch*);
+  int (*deri[12])(const QxArgs*, const QxLaunch*);  // array form (derivative tensor)
+  int (*grad[12])(const QxArgs*, const QxLaunch*);  // streaming adjoint gradient
   double eri_flops;          // algorithmic FP operations of one ERI body (no epilogue)
   double part_flops[12];     // same per derivative part
   int part_ncomb[12];        // derivative blocks (centre x direction) per part

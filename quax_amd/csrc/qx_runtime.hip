@@ -209,7 +209,9 @@ int launch_eri(qx_ctx* c, const QxWorkDev& w, int cls, const QxEpiDev& e, int fp
   QxArgs a;
   a.pr = pr; a.pp = pp; a.w = w; a.e = e; a.boys = c->d_boys;
   QxLaunch L{fp32, s};
-  return timed_launch(c, cls, w.nquart, s, [&] { return qx_class_table[cls].eri(&a, &L); });
+  const QxClassEntry& ent = qx_class_table[cls];
+  auto fn = e.epi == QX_EPI_JK ? ent.jk : ent.eri;
+  return timed_launch(c, cls, w.nquart, s, [&] { return fn(&a, &L); });
 }
 
 void check_launch(int rc, const char* what) {
@@ -615,7 +617,8 @@ static void launch_deriv_all(qx_ctx* c, const QxEpiDev& e, int fp32, hipStream_t
       a.pr = c->pr; a.pp = c->pp; a.w = wh.w; a.e = e; a.boys = c->d_boys;
       QxLaunch L{fp32, s};
       const int slot = QX_NUM_CLASSES + wh.cls * 12 + k;
-      check_launch(timed_launch(c, slot, wh.w.nquart, s, [&] { return ent.deri[k](&a, &L); }), "eri derivative");
+      auto fn = e.epi == QX_EPI_GRAD ? ent.grad[k] : ent.deri[k];
+      check_launch(timed_launch(c, slot, wh.w.nquart, s, [&] { return fn(&a, &L); }), "eri derivative");
     }
   }
 }

@@ -1,0 +1,71 @@
+// Runs the C ABI of libquax_hip on the host emulation (see hip/hip_runtime.h)
+// for one molecule read from a text file and writes every result as raw
+// doubles -- TEST INFRASTRUCTURE.
+//
+// input: natom; xyz (3*natom); Z (natom); nshell; per shell: l atom nprim;
+//        exps; coefs; pure tol; D (nbf*nbf, AO basis)
+// output (binary, in order): [G nbf^4 if nbf <= 60] dG (3N nbf^4 if nbf <= 40)
+//        J K (nbf^2 each) grad2e (3N) S T V (nbf^2 each) grad1e (3N)
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#include "quax_amd.h"
+
+static void die(qx_ctx* c, int rc, const char* what) {
+  std::fprintf(stderr, "%s failed (%d): %s\n", what, rc, qx_last_error(c));
+  std::exit(2);
+}
+
+int main(int argc, char** argv) {
+  if (argc < 3) return 1;
+  FILE* f = std::fopen(argv[1], "r");
+  int natom, nshell, pure;
+  double tol;
+  std::fscanf(f, "%d", &natom);
+  std::vector<double> xyz(3 * natom), Z(natom);
+  for (auto& v : xyz) std::fscanf(f, "%lf", &v);
+  for (auto& v : Z) std::fscanf(f, "%lf", &v);
+  std::fscanf(f, "%d", &nshell);
+  std::vector<int> l(nshell), at(nshell), np(nshell);
+  int tp = 0;
+  for (int s = 0; s < nshell; ++s) {
+    std::fscanf(f, "%d %d %d", &l[s], &at[s], &np[s]);
+    tp += np[s];
+  }
+  std::vector<double> ex(tp), co(tp);
+  for (auto& v : ex) std::fscanf(f, "%lf", &v);
+  for (auto& v : co) std::fscanf(f, "%lf", &v);
+  std::fscanf(f, "%d %lf", &pure, &tol);
+  qx_ctx* c = nullptr;
+  int rc = qx_initialize(natom, xyz.data(), Z.data(), nshell, l.data(), at.data(), np.data(), ex.data(), co.data(),
+                         pure, tol, 0, &c);
+  if (rc) die(nullptr, rc, "initialize");
+  int nbf, nc;
+  qx_nbf(c, &nbf, &nc);
+  std::vector<double> D((size_t)nbf * nbf);
+  for (auto& v : D) std::fscanf(f, "%lf", &v);
+  std::fclose(f);
+  FILE* o = std::fopen(argv[2], "wb");
+  const size_t n2 = (size_t)nbf * nbf, n4 = n2 * n2;
+  if (nbf <= 60) {
+    std::vector<double> G(n4);
+    if ((rc = qx_compute_2e_int(c, G.data(), nullptr))) die(c, rc, "compute_2e_int");
+    std::fwrite(G.data(), 8, G.size(), o);
+  }
+  if (nbf <= 40) {
+    std::vector<double> dG(3 * natom * n4);
+    if ((rc = qx_eri_deriv_core(c, 1, dG.data(), nullptr))) die(c, rc, "eri_deriv_core");
+    std::fwrite(dG.data(), 8, dG.size(), o);
+  }
+  std::vector<double> J(n2), K(n2), g(3 * natom, 0.0), S(n2), T(n2), V(n2), g1(3 * natom, 0.0);
+  if ((rc = qx_jk_build(c, D.data(), J.data(), K.data(), 0, 0.0, nullptr))) die(c, rc, "jk_build");
+  if ((rc = qx_eri_grad(c, D.data(), g.data(), 0, 0.0, nullptr))) die(c, rc, "eri_grad");
+  if ((rc = qx_oei(c, S.data(), T.data(), V.data(), nullptr))) die(c, rc, "oei");
+  if ((rc = qx_oei_grad(c, D.data(), D.data(), g1.data(), nullptr))) die(c, rc, "oei_grad");
+  for (auto* v : {&J, &K, &g, &S, &T, &V, &g1}) std::fwrite(v->data(), 8, v->size(), o);
+  std::fclose(o);
+  qx_finalize(c);
+  std::fprintf(stderr, "emu: ok\n");
+  return 0;
+}

@@ -1,0 +1,116 @@
+// Host emulation of the HIP subset libquax_hip uses -- TEST INFRASTRUCTURE.
+//
+// Lets the real runtime (qx_runtime.hip, qx_kernels.hip) and the generated
+// per-class kernels compile with g++ for the CPU, so that index arithmetic,
+// work decomposition and memory accesses of every kernel can be checked under
+// AddressSanitizer and compared with the CPU oracle without a GPU.
+//
+// Execution model: blocks run one after another; the lanes of a block run on a
+// persistent pool of OS threads (one per lane), so __syncthreads and
+// wavefront shuffles are real rendezvous.  A shuffle reached by only some
+// lanes deadlocks -- which is itself a finding (divergent cross-lane op).
+// Device memory is host heap memory (ASan-checked).
+#pragma once
+#include <atomic>
+#include <barrier>
+#include <cmath>
+#include <condition_variable>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <functional>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#define __global__
+#define __device__
+#define __host__
+#define __forceinline__ inline
+#define __launch_bounds__(...)
+#define __shared__ static
+
+typedef int hipError_t;
+typedef void* hipStream_t;
+typedef void* hipEvent_t;
+enum { hipSuccess = 0, hipErrorOutOfMemory = 2, hipErrorInvalidValue = 1 };
+enum hipMemcpyKind { hipMemcpyHostToDevice, hipMemcpyDeviceToHost, hipMemcpyDeviceToDevice, hipMemcpyHostToHost };
+
+struct dim3 {
+  unsigned x, y, z;
+  dim3(unsigned a = 1, unsigned b = 1, unsigned c = 1) : x(a), y(b), z(c) {}
+};
+
+struct EmuIdx {
+  unsigned x = 0, y = 0, z = 0;
+};
+extern thread_local EmuIdx threadIdx, blockIdx;
+extern thread_local dim3 blockDim, gridDim;
+
+inline const char* hipGetErrorString(hipError_t e) { return e ? "emulated HIP error" : "no error"; }
+inline hipError_t hipGetLastError() { return hipSuccess; }
+inline hipError_t hipSetDevice(int) { return hipSuccess; }
+inline hipError_t hipGetDeviceCount(int* n) {
+  *n = 1;
+  return hipSuccess;
+}
+inline hipError_t hipMalloc(void** p, size_t n) {
+  *p = std::malloc(n);
+  return *p ? hipSuccess : hipErrorOutOfMemory;
+}
+inline hipError_t hipFree(void* p) {
+  std::free(p);
+  return hipSuccess;
+}
+inline hipError_t hipMemcpy(void* d, const void* s, size_t n, hipMemcpyKind) {
+  std::memcpy(d, s, n);
+  return hipSuccess;
+}
+inline hipError_t hipDeviceSynchronize() { return hipSuccess; }
+inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
+inline hipError_t hipEventCreate(hipEvent_t* e) {
+  *e = std::malloc(8);
+  return hipSuccess;
+}
+inline hipError_t hipEventDestroy(hipEvent_t e) {
+  std::free(e);
+  return hipSuccess;
+}
+inline hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
+inline hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
+inline hipError_t hipEventElapsedTime(float* t, hipEvent_t, hipEvent_t) {
+  *t = 0.f;
+  return hipSuccess;
+}
+
+// ---- lane pool -----------------------------------------------------------
+namespace emu {
+void run_grid(dim3 grid, dim3 block, const std::function<void()>& body);
+double shfl_xor(double v, int off);
+int shfl_xor(int v, int off);
+unsigned long long ballot(bool p);
+void syncthreads();
+}  // namespace emu
+
+#define __syncthreads() emu::syncthreads()
+inline double __shfl_xor(double v, int off, int = 64) { return emu::shfl_xor(v, off); }
+inline int __shfl_xor(int v, int off, int = 64) { return emu::shfl_xor(v, off); }
+inline unsigned long long __ballot(bool p) { return emu::ballot(p); }
+inline unsigned __lane_id() { return threadIdx.x % 64; }
+inline int __ffsll(long long m) { return m ? __builtin_ffsll(m) : 0; }
+
+inline double unsafeAtomicAdd(double* p, double v) {
+  return std::atomic_ref<double>(*p).fetch_add(v);
+}
+inline double atomicAdd(double* p, double v) { return std::atomic_ref<double>(*p).fetch_add(v); }
+inline float atomicAdd(float* p, float v) { return std::atomic_ref<float>(*p).fetch_add(v); }
+
+template <class K, class... A>
+inline void emu_launch(K kernel, dim3 grid, dim3 block, A... args) {
+  emu::run_grid(grid, block, [=]() { kernel(args...); });
+}
+#define hipLaunchKernelGGL(kernel, grid, block, shmem, stream, ...) emu_launch(kernel, grid, block, __VA_ARGS__)
+
+using std::max;
+using std::min;
