@@ -41,29 +41,37 @@ def log(*a):
 
 
 def kernel_table(eng):
-    """Per profile slot: name, algorithmic flops per primitive quartet
-    (including the epilogue contraction)."""
+    """Per profile slot: kernel name and algorithmic flops per primitive
+    quartet (including the epilogue contraction)."""
     import ctypes
     L = eng.L
-    nc = L.qx_num_classes()
+    ncart = lambda l: (l + 1) * (l + 2) // 2
+    classes = {}
     info = {}
-    for k in range(nc):
-        l4 = (ctypes.c_int * 4)()
-        npart = ctypes.c_int()
-        L.qx_class_info(k, l4, ctypes.byref(npart))
-        ef = ctypes.c_double()
-        pf = (ctypes.c_double * 12)()
-        pc = (ctypes.c_int * 12)()
-        L.qx_class_flops(k, ctypes.byref(ef), pf, pc)
-        la, lb, lc, ld = l4[:]
-        ncart = lambda l: (l + 1) * (l + 2) // 2
-        N = ncart(la) * ncart(lb) * ncart(lc) * ncart(ld)
-        nm = f"{la}{lb}{lc}{ld}"
-        # J/K epilogue: 6 fused multiply-adds per integral
-        info[k] = (f"qx_kj_e{nm}", ef.value + 12.0 * N)
-        for p in range(npart.value):
-            # adjoint epilogue: density pair product (8 flops) + 2 per block
-            info[nc + 12 * k + p] = (f"qx_kg_d{nm}_{p}", pf[p] + 8.0 * N + 2.0 * pc[p] * N)
+    for slot in range(L.qx_profile_slots()):
+        kind, cls, part, f32 = (ctypes.c_int() for _ in range(4))
+        L.qx_profile_slot(slot, ctypes.byref(kind), ctypes.byref(cls), ctypes.byref(part), ctypes.byref(f32))
+        c = cls.value
+        if c not in classes:
+            l4 = (ctypes.c_int * 4)()
+            npart = ctypes.c_int()
+            L.qx_class_info(c, l4, ctypes.byref(npart))
+            ef = ctypes.c_double()
+            pf = (ctypes.c_double * 48)()
+            pc = (ctypes.c_int * 48)()
+            L.qx_class_flops(c, ctypes.byref(ef), pf, pc)
+            classes[c] = (tuple(l4), npart.value, ef.value, list(pf))
+        l4, npart, ef, pf = classes[c]
+        nm = "".join(map(str, l4))
+        tp = "float" if f32.value else "double"
+        N = ncart(l4[0]) * ncart(l4[1]) * ncart(l4[2]) * ncart(l4[3])
+        if kind.value == 0:
+            # J/K epilogue: 6 fused multiply-adds per integral
+            info[slot] = (f"qx_kj_e{nm}<{tp}>", ef + 12.0 * N)
+        elif part.value < npart:
+            info[slot] = (f"qx_kg_d{nm}_{part.value}<{tp}>", pf[part.value])
+        else:
+            info[slot] = (f"unused slot {slot}", 0.0)
     return info
 
 
@@ -141,6 +149,52 @@ def cpu_baseline(mol, basis, tol, eng, seconds):
     return res, nthreads
 
 
+def hf_energy_gradient(eng, mol, precision, thr):
+    """RHF energy (DIIS, conv 1e-10) + analytic gradient; wall times."""
+    import torch
+    from quax_amd.methods.hartree_fock import restricted_hartree_fock, rhf_gradient
+    opts = {"diis": True, "spectral_shift": False, "conv": 1e-10, "maxit": 200, "precision": precision,
+            "fp32_threshold": thr}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    E, aux = restricted_hartree_fock(eng, mol.nelectron(), opts, return_aux_data=True)
+    torch.cuda.synchronize()
+    t_scf = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    grad = rhf_gradient(eng, aux, precision=precision, fp32_threshold=thr)
+    torch.cuda.synchronize()
+    return E, aux, grad, t_scf, time.perf_counter() - t0
+
+
+def timed_steps(eng, D, steps, warmup, precision, thr, profile):
+    """warmup + steps of (J/K pass, adjoint gradient pass); returns seconds
+    and, when profile is set, the per-slot HIP-event kernel times."""
+    import torch
+    J = torch.empty_like(D)
+    K = torch.empty_like(D)
+
+    def step():
+        eng.jk(D, precision=precision, fp32_threshold=thr, out=(J, K))
+        eng.eri_grad(D, precision=precision, fp32_threshold=thr)
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    if profile:
+        eng.profile(True)
+    t0 = time.perf_counter()
+    for i in range(steps):
+        step()
+        log(f"[bench] {'fp64' if precision == 0 else 'mixed'} step {i + 1}/{steps}")
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    prof = None
+    if profile:
+        prof = eng.profile_read()
+        eng.profile(False)
+    return dt, prof
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -148,6 +202,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU baseline budget (0 = skip)")
     ap.add_argument("--workload", default="cluster10", choices=["cluster10", "cluster24", "water"])
+    ap.add_argument("--fp32-threshold", type=float, default=1e-8,
+                    help="Schwarz bound below which the mixed runs use FP32 quartets")
+    ap.add_argument("--streams", type=int, default=4, help="streams per pass (1 = every kernel timed alone)")
+    ap.add_argument("--no-scf", action="store_true",
+                    help="skip the SCF and use the core-Hamiltonian density (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is design-only)",
@@ -159,12 +218,12 @@ def main():
     from quax_amd import workloads
     from quax_amd.basis import BasisSet
     from quax_amd.engine import EriEngine
-    from quax_amd.methods.hartree_fock import restricted_hartree_fock, rhf_gradient
 
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
     tol = 1e-12
+    thr = args.fp32_threshold
     if args.workload == "water":
         mol, bname, pure = workloads.water_sto3g(), "sto-3g", False
     elif args.workload == "cluster24":
@@ -174,63 +233,56 @@ def main():
     basis = BasisSet.build(mol, bname, pure=pure)
     t_init = time.perf_counter()
     eng = EriEngine(mol, basis, ints_tol=tol)
+    eng.set_streams(args.streams)
     torch.cuda.synchronize()
     t_init = time.perf_counter() - t_init
     cq = eng.class_quartets()
     q_eri = int(sum(x[1] for x in cq))
     q_deri = int(sum(x[2] for x in cq))
+    n64, n32 = eng.mixed_quartets(thr)
     log(f"[bench] {args.workload}: nbf {eng.nbf} (cart {eng.ncart}), ERI quartets {q_eri}, dERI quartets {q_deri}, "
-        f"init {t_init:.2f} s")
+        f"init {t_init:.2f} s; mixed @ {thr:g}: {n32 / max(n64 + n32, 1):.1%} of quartets in FP32")
 
-    # HF energy + gradient (FP64), wall time end to end after set-up
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    E, aux = restricted_hartree_fock(eng, mol.nelectron(), {"diis": True, "spectral_shift": False, "conv": 1e-10,
-                                                            "maxit": 200}, return_aux_data=True)
-    torch.cuda.synchronize()
-    t_scf = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    grad = rhf_gradient(eng, aux)
-    torch.cuda.synchronize()
-    t_grad = time.perf_counter() - t0
-    log(f"[bench] RHF E = {E:.10f} in {aux['iterations']} iterations ({t_scf:.2f} s), gradient {t_grad:.3f} s, "
-        f"|sum g| = {abs(grad.reshape(-1, 3).sum(0)).max():.2e}")
-    D = aux["D"].contiguous()
-    J = torch.empty_like(D)
-    K = torch.empty_like(D)
+    hf = {"fp32_threshold": thr, "fp32_quartet_fraction": n32 / max(n64 + n32, 1)}
+    if args.no_scf:
+        S, T, V = eng.oei()
+        A = torch.linalg.inv(torch.linalg.cholesky(S)).T
+        _, C2 = torch.linalg.eigh(A.T @ (T + V) @ A)
+        Cocc = (A @ C2)[:, :mol.nelectron() // 2]
+        D = (Cocc @ Cocc.T).contiguous()
+    else:
+        E, aux, grad, t_scf, t_grad = hf_energy_gradient(eng, mol, "fp64", thr)
+        log(f"[bench] FP64 RHF E = {E:.10f} in {aux['iterations']} iterations ({t_scf:.2f} s), "
+            f"gradient {t_grad:.3f} s, |sum g| = {abs(grad.reshape(-1, 3).sum(0)).max():.2e}")
+        Em, auxm, gradm, t_scfm, t_gradm = hf_energy_gradient(eng, mol, "mixed", thr)
+        log(f"[bench] mixed RHF E = {Em:.10f} (error {Em - E:.2e}) in {auxm['iterations']} iterations "
+            f"({t_scfm:.2f} s), gradient {t_gradm:.3f} s, max gradient error {np.abs(gradm - grad).max():.2e}")
+        D = aux["D"].contiguous()
+        hf.update({"energy": E, "scf_iterations": aux["iterations"], "scf_wall_s": t_scf, "grad_wall_s": t_grad,
+                   "hf_gradient_wall_s_fp64": t_scf + t_grad,
+                   "hf_gradient_wall_s_mixed": t_scfm + t_gradm, "scf_iterations_mixed": auxm["iterations"],
+                   "energy_error_mixed": Em - E, "grad_max_error_mixed": float(np.abs(gradm - grad).max()),
+                   "speedup_mixed": (t_scf + t_grad) / (t_scfm + t_gradm)})
 
-    def step():
-        eng.jk(D, out=(J, K))
-        eng.eri_grad(D)
-
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    eng.profile(True)
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        step()
-        log(f"[bench] step {i + 1}/{args.steps}")
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    ms, launches, pq = eng.profile_read()
-    eng.profile(False)
+    # headline: FP64 steps (J/K pass + adjoint gradient pass), kernels timed with HIP events
+    dt, (ms, launches, pq) = timed_steps(eng, D, args.steps, args.warmup, 0, 0.0, True)
     ms_step = dt * 1e3 / args.steps
     value = (q_eri + q_deri) / (dt / args.steps)
+    dtm, _ = timed_steps(eng, D, args.steps, 1, 1, thr, False)
+    ms_step_mixed = dtm * 1e3 / args.steps
 
-    # dominant kernel and its roofline (FP64 compute bound)
     table = kernel_table(eng)
-    nc = eng.L.qx_num_classes()
-    eri_ms = float(ms[:nc].sum()) / args.steps
-    deri_ms = float(ms[nc:].sum()) / args.steps
+    eri_slots = [k for k in range(len(ms)) if table[k][0].startswith("qx_kj")]
+    eri_ms = float(ms[eri_slots].sum()) / args.steps
+    deri_ms = float(ms.sum()) / args.steps - eri_ms
     slot = int(np.argmax(ms))
     name, fl = table[slot]
     avg_ms = ms[slot] / launches[slot]
     flops_per_launch = fl * pq[slot] / launches[slot]
     achieved = flops_per_launch / (avg_ms * 1e-3)
-    total_flops = sum(table[s][1] * pq[s] for s in range(len(ms)) if launches[s]) / args.steps
-    kernels = sorted(((float(ms[s]) / args.steps, table[s][0], int(pq[s] // max(launches[s], 1)))
-                      for s in range(len(ms)) if launches[s]), reverse=True)[:8]
+    total_flops = sum(table[k][1] * pq[k] for k in range(len(ms)) if launches[k]) / args.steps
+    kernels = sorted(((float(ms[k]) / args.steps, table[k][0], int(pq[k] // max(launches[k], 1)))
+                      for k in range(len(ms)) if launches[k]), reverse=True)[:8]
 
     cpu = None
     if args.cpu_seconds > 0:
@@ -263,14 +315,16 @@ def main():
         "config": {"workload": f"{args.workload}: {mol.natom()} atoms, {bname} "
                                f"{'spherical' if pure else 'Cartesian'} basis ({eng.nbf} AO / {eng.ncart} Cartesian), "
                                f"Schwarz 1e-12; step = direct J/K ERI pass + adjoint dERI gradient pass",
-                   "eri_quartets": q_eri, "deri_quartets": q_deri, "eri_ms": eri_ms, "deri_ms": deri_ms},
+                   "eri_quartets": q_eri, "deri_quartets": q_deri, "streams": args.streams,
+                   "eri_kernel_ms": eri_ms, "deri_kernel_ms": deri_ms},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64, "traffic": None, "kernel": name, "avg_launch_ms": avg_ms,
                      "flops_per_launch": flops_per_launch, "step_flops": total_flops,
                      "step_tflops": total_flops / (ms_step * 1e-3) / 1e12},
         "cpu_baseline": cpu,
-        "hf": {"energy": E, "scf_iterations": aux["iterations"], "scf_wall_s": t_scf, "grad_wall_s": t_grad,
-               "hf_gradient_wall_s_fp64": t_scf + t_grad, "hf_gradient_wall_s_mixed": None},
+        "mixed": {"fp32_threshold": thr, "ms_per_step": ms_step_mixed,
+                  "value": (q_eri + q_deri) / (dtm / args.steps), "speedup_vs_fp64_step": ms_step / ms_step_mixed},
+        "hf": hf,
         "top_kernels_ms_per_step": kernels,
     }
     print(json.dumps(out), flush=True)

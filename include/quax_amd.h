@@ -96,8 +96,11 @@ long long qx_generate_multi_index_lookup(int nparams, int deriv_order, int* out,
 /* Integral-direct J and K (hartree_fock.py:25-28, 79-80 contract the stored
  * tensor the same way): J_pq = sum_rs (pq|rs) D_rs, K_pq = sum_rs (pr|qs) D_rs.
  * D, J, K: nbf x nbf device buffers.  precision QX_PREC_MIXED evaluates
- * quartets whose Schwarz bound sqrt(Q_ab Q_cd) is below fp32_threshold with
- * an FP32 recursion, summed in FP64. */
+ * quartets whose Schwarz bound sqrt(Q_ab Q_cd) is below fp32_threshold
+ * (finite, >= 0) with an FP32 recursion (Boys function and quartet set-up in
+ * FP64), each integral contracted into FP64 accumulators.  The split lists
+ * are rebuilt when the threshold changes (that call synchronises the
+ * device). */
 int qx_jk_build(qx_ctx* ctx, const double* D, double* J, double* K, int precision, double fp32_threshold,
                 void* stream);
 
@@ -128,23 +131,37 @@ int qx_oei_deriv(qx_ctx* ctx, double* dS, double* dT, double* dV, void* stream);
 int qx_store_size(const qx_ctx* ctx, long long* ndoubles);
 int qx_eri_store(qx_ctx* ctx, double* out, void* stream);
 
+/* Mixed-precision split of the canonical screened quartets at
+ * fp32_threshold: contracted quartets evaluated in FP64 and in FP32. */
+int qx_mixed_quartets(qx_ctx* ctx, double fp32_threshold, long long* n_fp64, long long* n_fp32);
+
+/* Independent class kernels of a J/K or gradient pass are spread over
+ * nstreams streams (1..8; default 4): the caller's stream plus side streams
+ * of default priority that join back into it before the call returns. */
+int qx_set_streams(qx_ctx* ctx, int nstreams);
+
 /* Measurement support (no reference counterpart).
  * qx_class_quartets: canonical screened quartets per kernel class (eri[]) and
  * those not on a single atom (deri[], the derivative work); arrays of
  * qx_num_classes() entries.  qx_class_info: angular momenta (l[4]) and the
  * number of derivative kernel parts of class cls.
  * qx_profile_enable(1) resets and starts recording a HIP event pair around
- * every kernel launch; qx_profile_read synchronises and returns, per slot
- * (ERI class c -> c; derivative part k of class c -> num_classes + 12c + k),
- * the summed kernel time in ms, the launch count and the primitive-pair
- * quartets processed.  Arrays hold qx_profile_slots() entries. */
+ * every kernel launch, on the stream the kernel runs on (with
+ * qx_set_streams(ctx, 1) every kernel is timed alone); qx_profile_read
+ * synchronises and returns, per
+ * slot, the summed kernel time in ms, the launch count and the primitive-pair
+ * quartets processed.  Arrays hold qx_profile_slots() entries;
+ * qx_profile_slot names a slot: kind 0 = ERI / J-K kernel of class cls,
+ * kind 1 = derivative part `part` of class cls; fp32 = FP32 variant. */
 int qx_num_classes(void);
 int qx_profile_slots(void);
+int qx_profile_slot(int slot, int* kind, int* cls, int* part, int* fp32);
 int qx_class_info(int cls, int* l4, int* nparts);
 int qx_class_quartets(const qx_ctx* ctx, long long* eri, long long* deri);
 /* Algorithmic FP operations (mul/add 1, fma 2) of one primitive-quartet ERI
- * body of class cls and of each derivative part, epilogue excluded, and the
- * number of derivative blocks each part produces. */
+ * body of class cls (J/K epilogue excluded) and of each derivative part
+ * (adjoint-gradient epilogue included), and the number of derivative blocks
+ * each part produces.  parts / ncomb hold up to 48 entries. */
 int qx_class_flops(int cls, double* eri, double* parts, int* ncomb);
 int qx_profile_enable(qx_ctx* ctx, int on);
 int qx_profile_read(qx_ctx* ctx, double* ms, long long* launches, long long* pquartets);

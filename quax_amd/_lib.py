@@ -56,6 +56,9 @@ SIGNATURES = {
     "qx_eri_store": (_i, [_vp, _vp, _vp]),
     "qx_num_classes": (_i, []),
     "qx_profile_slots": (_i, []),
+    "qx_profile_slot": (_i, [_i, _ip, _ip, _ip, _ip]),
+    "qx_mixed_quartets": (_i, [_vp, ctypes.c_double, ctypes.POINTER(_ll), ctypes.POINTER(_ll)]),
+    "qx_set_streams": (_i, [_vp, _i]),
     "qx_class_info": (_i, [_i, _ip, _ip]),
     "qx_class_quartets": (_i, [_vp, ctypes.POINTER(_ll), ctypes.POINTER(_ll)]),
     "qx_class_flops": (_i, [_i, _dp, _dp, _ip]),

@@ -5,7 +5,7 @@ compile every translation unit with hipcc in parallel, link the shared
 library into quax_amd/lib/.  Objects are cached under csrc/_build keyed by a
 hash of (source, headers, flags); ``force=True`` recompiles everything.
 
-Usage: python -m quax_amd.csrc.build [--force] [--jobs N] [--lmax L] [--fp32]
+Usage: python -m quax_amd.csrc.build [--force] [--jobs N] [--lmax L] [--no-fp32]
 """
 from __future__ import annotations
 
@@ -66,10 +66,10 @@ def _compile(src, force):
     return obj, time.time() - t0, False
 
 
-def build(force=False, jobs=None, lmax=2, fp32=False, verbose=True):
+def build(force=False, jobs=None, lmax=2, fp32=True, verbose=True):
     os.makedirs(OBJ, exist_ok=True)
     os.makedirs(LIBDIR, exist_ok=True)
-    gen_args = [sys.executable, os.path.join(HERE, "gen_eri.py"), GEN, str(lmax)] + (["--fp32"] if fp32 else [])
+    gen_args = [sys.executable, os.path.join(HERE, "gen_eri.py"), GEN, str(lmax)] + ([] if fp32 else ["--no-fp32"])
     if os.path.isdir(GEN):
         shutil.rmtree(GEN)
     subprocess.run(gen_args, check=True, capture_output=True)
@@ -100,6 +100,6 @@ if __name__ == "__main__":
     ap.add_argument("--force", action="store_true")
     ap.add_argument("--jobs", type=int, default=None)
     ap.add_argument("--lmax", type=int, default=2)
-    ap.add_argument("--fp32", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 (mixed-precision) kernels")
     a = ap.parse_args()
-    build(force=a.force, jobs=a.jobs, lmax=a.lmax, fp32=a.fp32)
+    build(force=a.force, jobs=a.jobs, lmax=a.lmax, fp32=not a.no_fp32)

@@ -14,13 +14,17 @@ the epilogue), as straight-line code:
      diagonal, direct J/K).
 
 Derivative kernels build d/dX_i (ab|cd) from d/dA_i phi_a = 2 alpha
-phi_{a+1_i} - a_i phi_{a-1_i}.  Light classes use one kernel that forms
-d/dA, d/dB, d/dC and infers d/dD by translational invariance; heavy classes
-are split into parts (one centre, or one centre and one Cartesian direction,
-per kernel) so that no single straight-line body becomes too large to
-register-allocate.  Epilogues: full-tensor scatter or the adjoint gradient.
+phi_{a+1_i} - a_i phi_{a-1_i}.  Every kernel forms d/dA, d/dB, d/dC and
+infers d/dD by translational invariance.  A class whose body would exceed
+DERIV_NODES recurrence nodes is split into parts over the Cartesian
+components of a (then of b): each part evaluates only the output functions
+(a, b, c, d) with a (b) in its range.  A part recomputes the recurrence
+nodes it shares with the others, but its live set fits the register file
+(the unsplit heavy bodies spill several KB per lane to scratch, and scratch
+latency, not arithmetic, then bounds them).  Epilogues: full-tensor scatter
+or the adjoint gradient.
 
-Usage: python gen_eri.py OUTDIR [LMAX]
+Usage: python gen_eri.py OUTDIR [LMAX] [--no-fp32]
 """
 from __future__ import annotations
 
@@ -29,7 +33,7 @@ import sys
 
 AX = "xyz"
 # largest SSA node count allowed in one derivative body before it is split
-SPLIT_NODES = 7000
+DERIV_NODES = int(os.environ.get("QX_DERIV_NODES", "2000"))
 
 
 def comps(l):
@@ -198,14 +202,17 @@ CENTRES = "ABCD"
 
 def deriv_targets(l, part):
     """Targets of one derivative part, in emission order: for every
-    (a,b,c,d), for every (centre, x) of the part: raised, then lowered."""
+    (a,b,c,d) of the part's a/b ranges, for every (centre, x): raised, then
+    lowered."""
     la, lb, lc, ld = l
+    a0, a1 = part["a"]
+    b0, b1 = part["b"]
     T = []
-    for a in comps(la):
-        for b in comps(lb):
+    for a in comps(la)[a0:a1]:
+        for b in comps(lb)[b0:b1]:
             for c in comps(lc):
                 for d in comps(ld):
-                    for (cen, x) in part:
+                    for (cen, x) in part["combos"]:
                         q = [a, b, c, d]
                         k = CENTRES.index(cen)
                         up = list(q)
@@ -218,22 +225,40 @@ def deriv_targets(l, part):
     return T
 
 
-def deriv_parts(l):
-    """Choose how to split the derivative of class l into kernels."""
-    full = [(c, x) for c in "ABC" for x in range(3)]
-    h, v, _, _, _ = build_dag(deriv_targets(l, full))
-    if len(h.lines) + len(v.lines) <= SPLIT_NODES:
-        return [("ABC", "xyz", True, full)]
-    parts = []
-    for c in CENTRES:
-        pc = [(c, x) for x in range(3)]
-        h, v, _, _, _ = build_dag(deriv_targets(l, pc))
-        if len(h.lines) + len(v.lines) <= SPLIT_NODES:
-            parts.append((c, "xyz", False, pc))
-        else:
-            for x in range(3):
-                parts.append((c, AX[x], False, [(c, x)]))
-    return parts
+def _ranges(n, k):
+    """k contiguous, nearly equal index ranges covering 0..n-1."""
+    return [(i * n // k, (i + 1) * n // k) for i in range(k)]
+
+
+def deriv_parts(l, budget=None):
+    """Split the derivative of class l into kernels of at most `budget`
+    recurrence nodes: whole class, else contiguous ranges of a components,
+    else one a component with ranges of b components (one (a, b) pair per
+    part is the finest split)."""
+    budget = budget or DERIV_NODES
+    la, lb = l[0], l[1]
+    NA, NB = ncart(la), ncart(lb)
+    combos = [(c, x) for c in "ABC" for x in range(3)]
+
+    def part(ar, br):
+        return {"combos": combos, "dinv": True, "a": ar, "b": br}
+
+    def nodes(p):
+        h, v, _, _, _ = build_dag(deriv_targets(l, p))
+        return len(h.lines) + len(v.lines)
+
+    whole = part((0, NA), (0, NB))
+    if nodes(whole) <= budget:
+        return [whole]
+    for k in range(2, NA + 1):
+        ps = [part(r, (0, NB)) for r in _ranges(NA, k)]
+        if all(nodes(p) <= budget for p in ps):
+            return ps
+    for k in range(1, NB + 1):
+        ps = [part((a, a + 1), r) for a in range(NA) for r in _ranges(NB, k)]
+        if k == NB or all(nodes(p) <= budget for p in ps):
+            return ps
+    raise AssertionError("unreachable")
 
 
 def emit_setup(L, Lt, deriv_w):
@@ -342,8 +367,10 @@ def emit_deriv_core(l, pi, part):
     nn = (ncart(la), ncart(lb), ncart(lc), ncart(ld))
     NA, NB, NC_, ND = nn
     N = NA * NB * NC_ * ND
-    cen, xs, dinv, combos = part
-    T = deriv_targets(l, combos)
+    combos, dinv = part["combos"], part["dinv"]
+    a0, a1 = part["a"]
+    b0, b1 = part["b"]
+    T = deriv_targets(l, part)
     h, v, vn, outs, pos = build_dag(T)
     Lt = sum(l) + 1
     nm = "".join(map(str, l))
@@ -353,11 +380,12 @@ def emit_deriv_core(l, pi, part):
     # per element: expressions of each combo and the position after which all exist
     elems = []
     ti = 0
-    e = 0
-    for a in comps(la):
-        for b in comps(lb):
-            for c in comps(lc):
-                for d in comps(ld):
+    for ia, a in enumerate(comps(la)[a0:a1], a0):
+        for ib, b in enumerate(comps(lb)[b0:b1], b0):
+            for ic, c in enumerate(comps(lc)):
+                for id_, d in enumerate(comps(ld)):
+                    e = ((ia * NB + ib) * NC_ + ic) * ND + id_
+                    le = (((ia - a0) * (b1 - b0) + (ib - b0)) * NC_ + ic) * ND + id_
                     q = [a, b, c, d]
                     exprs = []
                     last = 0
@@ -373,14 +401,14 @@ def emit_deriv_core(l, pi, part):
                             last = max(last, pos[ti])
                             ti += 1
                         exprs.append(f"w{cn} * {up}" + (f" - R({n}) * {dn}" if dn else ""))
-                    ijkl = ((["abcd".index(ch) for ch in "abcd"]))
-                    elems.append((e, last, exprs, (a, b, c, d)))
-                    e += 1
-    # array form
+                    elems.append((e, le, last, exprs))
+    # array form: dv holds only this part's elements (local index le)
+    NEL = len(elems)
     L = []
-    _head(L, f"qx_core_d{nm}_{pi}", f", R (&dv)[{NC * N}]", Lt, v, vn)
+    _head(L, f"qx_core_d{nm}_{pi}", f", R (&dv)[{NC * NEL}]", Lt, v, vn)
     L += wdecl
-    groups = [(last, [f"  dv[{ci * N + e}] = {ex};" for ci, ex in enumerate(exprs)]) for (e, last, exprs, _) in elems]
+    groups = [(last, [f"  dv[{ci * NEL + le}] = {ex};" for ci, ex in enumerate(exprs)])
+              for (e, le, last, exprs) in elems]
     L += ["  " + x if not x.startswith("  ") else x for x in interleave(h.lines, groups)]
     L.append("}")
     # streaming adjoint form
@@ -389,7 +417,7 @@ def emit_deriv_core(l, pi, part):
     G += wdecl
     emit_dblocks(G, nn)
     groups = []
-    for (e, last, exprs, _) in elems:
+    for (e, le, last, exprs) in elems:
         i, rem = divmod(e, NB * NC_ * ND)
         j, rem = divmod(rem, NC_ * ND)
         kk, ll = divmod(rem, ND)
@@ -403,10 +431,11 @@ def emit_deriv_core(l, pi, part):
     G.append("}")
     cmask = sum(1 << CENTRES.index(c) for c in set(cc for cc, _ in combos))
     xmask = sum(1 << x for x in set(x for _, x in combos))
-    stats = dict(part=pi, centres=cen, xs=xs, dinv=dinv, vrr=len(v.lines), vrr_flops=v.nflop,
-                 hrr=len(h.lines), hrr_flops=h.nflop, L=Lt, N=N, ncomb=NC,
-                 flops=core_flops(Lt, v.nflop, h.nflop) + 3 * NC * N)
-    return "\n".join(L), "\n".join(G), stats, (cmask, xmask, dinv, NC)
+    nel = len(elems)
+    stats = dict(part=pi, a=part["a"], b=part["b"], dinv=dinv, vrr=len(v.lines), vrr_flops=v.nflop,
+                 hrr=len(h.lines), hrr_flops=h.nflop, L=Lt, N=N, nel=nel, ncomb=NC,
+                 flops=core_flops(Lt, v.nflop, h.nflop) + nel * (8 + 5 * NC))
+    return "\n".join(L), "\n".join(G), stats, (cmask, xmask, dinv, NC, a0, a1, b0, b1)
 
 
 def core_flops(L, vrr_flops, hrr_flops):
@@ -449,9 +478,9 @@ def emit_class_files(cls, outdir, fp32):
     stats.append((cls, "eri", st))
     parts = deriv_parts(cls)
     for pi, part in enumerate(parts):
-        src_d, src_g, st, (cmask, xmask, dinv, NC) = emit_deriv_core(cls, pi, part)
+        src_d, src_g, st, (cmask, xmask, dinv, NC, a0, a1, b0, b1) = emit_deriv_core(cls, pi, part)
         args = f"{N}, {dims}, {cmask}, {xmask}, {1 if dinv else 0}, {NC}"
-        write(f"cls_{nm}_d{pi}.hip", [src_d, "", f"QX_DEFINE_DERI_KERNEL(d{nm}_{pi}, {args})"])
+        write(f"cls_{nm}_d{pi}.hip", [src_d, "", f"QX_DEFINE_DERI_KERNEL(d{nm}_{pi}, {args}, {a0}, {a1}, {b0}, {b1})"])
         write(f"cls_{nm}_g{pi}.hip", [src_g, "", f"QX_DEFINE_GRAD_KERNEL(d{nm}_{pi}, {args}, {1 if fp32 else 0})"])
         stats.append((cls, f"deriv{pi}", st))
     return files, stats, len(parts)
@@ -460,7 +489,7 @@ def emit_class_files(cls, outdir, fp32):
 def main():
     outdir = sys.argv[1]
     lmax = int(sys.argv[2]) if len(sys.argv) > 2 else 2
-    fp32 = "--fp32" in sys.argv
+    fp32 = "--no-fp32" not in sys.argv
     os.makedirs(outdir, exist_ok=True)
     _, classes = canonical_classes(lmax)
     files, allstats, nparts, flops = [], [], {}, {}
@@ -472,7 +501,8 @@ def main():
         flops[cls] = (s[0][2]["flops"], [st["flops"] for _, _, st in s[1:]], [st["ncomb"] for _, _, st in s[1:]])
     D = ["// generated: kernel launch tables indexed by canonical class id", "#pragma once",
          f"#define QX_GEN_LMAX {lmax}", f"#define QX_NUM_CLASSES {len(classes)}",
-         f"#define QX_MAX_DPARTS {max(nparts.values())}", ""]
+         f"#define QX_MAX_DPARTS {max(nparts.values())}",
+         "static_assert(QX_MAX_DPARTS <= QX_DPARTS_CAP, \"derivative part table too small\");", ""]
     for cls in classes:
         nm = "".join(map(str, cls))
         D.append(f'extern "C" int qx_launch_e{nm}(const QxArgs*, const QxLaunch*);')

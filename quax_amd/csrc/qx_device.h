@@ -305,10 +305,18 @@ __device__ __forceinline__ bool qx_single_atom(const QxArgs& A_, int P1, int P2)
   return a == A_.pr.atom_b[P1] && a == A_.pr.atom_a[P2] && a == A_.pr.atom_b[P2];
 }
 
-template <int NA, int NB, int NC, int ND, int CMASK, int XMASK, int DINV, int NCOMB, typename R>
-__device__ __forceinline__ void qx_epi_deri(const QxArgs& A_, int P1, int P2, const R (&dv)[NCOMB * NA * NB * NC * ND],
-                                            double (&gv)[12]) {
-  constexpr int N = NA * NB * NC * ND;
+// A part covers the output functions with a in [A0, A1) and b in [B0, B1):
+// dv holds NCOMB blocks of that part's NEL elements, in (a, b, c, d) order.
+template <int NA, int NB, int NC, int ND, int A0, int A1, int B0, int B1>
+struct QxPartShape {
+  static constexpr int NEL = (A1 - A0) * (B1 - B0) * NC * ND;
+};
+
+template <int NA, int NB, int NC, int ND, int CMASK, int XMASK, int DINV, int NCOMB, int A0, int A1, int B0, int B1,
+          typename R>
+__device__ __forceinline__ void qx_epi_deri(const QxArgs& A_, int P1, int P2,
+                                            const R (&dv)[NCOMB * QxPartShape<NA, NB, NC, ND, A0, A1, B0, B1>::NEL]) {
+  constexpr int NEL = QxPartShape<NA, NB, NC, ND, A0, A1, B0, B1>::NEL;
   const int at[4] = {A_.pr.atom_a[P1], A_.pr.atom_b[P1], A_.pr.atom_a[P2], A_.pr.atom_b[P2]};
   const int n = A_.e.nbf;
   const int ia = A_.pr.bfa[P1], ib = A_.pr.bfb[P1], ic = A_.pr.bfa[P2], id = A_.pr.bfb[P2];
@@ -318,14 +326,14 @@ __device__ __forceinline__ void qx_epi_deri(const QxArgs& A_, int P1, int P2, co
     const bool sab = A_.pr.sa[P1] == A_.pr.sb[P1], scd = A_.pr.sa[P2] == A_.pr.sb[P2], sp = P1 == P2;
     const int coord = A_.e.coord;
 #pragma unroll
-    for (int i = 0; i < NA; ++i)
+    for (int i = A0; i < A1; ++i)
 #pragma unroll
-      for (int j = 0; j < NB; ++j)
+      for (int j = B0; j < B1; ++j)
 #pragma unroll
         for (int k = 0; k < NC; ++k)
 #pragma unroll
           for (int l = 0; l < ND; ++l) {
-            const int e = ((i * NB + j) * NC + k) * ND + l;
+            const int e = (((i - A0) * (B1 - B0) + (j - B0)) * NC + k) * ND + l;
             double dsum[3] = {0.0, 0.0, 0.0};
             int ci = 0;
 #pragma unroll
@@ -334,7 +342,7 @@ __device__ __forceinline__ void qx_epi_deri(const QxArgs& A_, int P1, int P2, co
 #pragma unroll
               for (int x = 0; x < 3; ++x) {
                 if (!(XMASK & (1 << x))) continue;
-                const double v = (double)dv[(ci++) * N + e];
+                const double v = (double)dv[(ci++) * NEL + e];
                 dsum[x] += v;
                 const int slot = 3 * at[cen] + x;
                 if (coord >= 0 && slot != coord) continue;
@@ -408,17 +416,18 @@ __device__ __forceinline__ void qx_epi_deri(const QxArgs& A_, int P1, int P2, co
   }
 
 // Array-form derivative kernel: full derivative-tensor scatter.
-#define QX_DEFINE_DERI_KERNEL(NAME, N, NA, NB, NC, ND, CMASK, XMASK, DINV, NCOMB)                 \
+#define QX_DEFINE_DERI_KERNEL(NAME, N, NA, NB, NC, ND, CMASK, XMASK, DINV, NCOMB, A0, A1, B0, B1) \
   __global__ __launch_bounds__(QX_BLOCK) void qx_k_##NAME(QxArgs A_) {                            \
     const long long t = (long long)blockIdx.x * QX_BLOCK + threadIdx.x;                          \
     if (t >= A_.w.nquart) return;                                                                 \
     int u1, u2, P1, P2;                                                                           \
     qx_locate(A_, t, u1, u2, P1, P2);                                                             \
     if (qx_single_atom(A_, P1, P2)) return;                                                       \
-    double dv[NCOMB * N];                                                                         \
+    static_assert(NCOMB * QxPartShape<NA, NB, NC, ND, A0, A1, B0, B1>::NEL * 8 <= 16384,           \
+                  "per-lane derivative block must stay small (scratch is allocated per lane)");    \
+    double dv[NCOMB * QxPartShape<NA, NB, NC, ND, A0, A1, B0, B1>::NEL];                          \
     qx_core_##NAME<double>(A_, P1, P2, u1, u2, dv);                                               \
-    double gv[12];                                                                                \
-    qx_epi_deri<NA, NB, NC, ND, CMASK, XMASK, DINV, NCOMB, double>(A_, P1, P2, dv, gv);           \
+    qx_epi_deri<NA, NB, NC, ND, CMASK, XMASK, DINV, NCOMB, A0, A1, B0, B1, double>(A_, P1, P2, dv); \
   }                                                                                               \
   extern "C" int qx_launch_##NAME(const QxArgs* a, const QxLaunch* L) {                           \
     if (a->w.nquart <= 0) return 0;                                                               \
@@ -478,14 +487,16 @@ __device__ __forceinline__ void qx_epi_deri(const QxArgs& A_, int P1, int P2, co
     return (int)hipGetLastError();                                                                \
   }
 
+#define QX_DPARTS_CAP 48  // most derivative parts one class may be split into
+
 struct QxClassEntry {
   int la, lb, lc, ld;
   int (*eri)(const QxArgs*, const QxLaunch*);      // array form (store, scatter, Schwarz)
   int (*jk)(const QxArgs*, const QxLaunch*);       // streaming J/K
   int nparts;
-  int (*deri[12])(const QxArgs*, const QxLaunch*);  // array form (derivative tensor)
-  int (*grad[12])(const QxArgs*, const QxLaunch*);  // streaming adjoint gradient
-  double eri_flops;          // algorithmic FP operations of one ERI body (no epilogue)
-  double part_flops[12];     // same per derivative part
-  int part_ncomb[12];        // derivative blocks (centre x direction) per part
+  int (*deri[QX_DPARTS_CAP])(const QxArgs*, const QxLaunch*);  // array form (derivative tensor)
+  int (*grad[QX_DPARTS_CAP])(const QxArgs*, const QxLaunch*);  // streaming adjoint gradient
+  double eri_flops;                  // algorithmic FP operations of one ERI body + J/K epilogue
+  double part_flops[QX_DPARTS_CAP];  // same per derivative part, with the adjoint epilogue
+  int part_ncomb[QX_DPARTS_CAP];     // derivative blocks (centre x direction) per part
 };

@@ -80,12 +80,28 @@ struct qx_ctx {
   std::vector<hipEvent_t> event_pool;
   std::vector<double> prof_ms;
   std::vector<long long> prof_launches, prof_pq;
+
+  // independent class kernels fan out over this many streams (the caller's
+  // stream plus side streams created on first use, default priority)
+  int nstreams = 4;
+  std::vector<hipStream_t> side;
+  std::vector<hipEvent_t> fork_ev;     // [0] fork, [1..] joins
+
+  // mixed precision: quartet lists split at one Schwarz-bound threshold
+  struct Split {
+    double thr = -1.0;
+    std::vector<QxWorkHost> w64, w32;  // FP64 (bound >= thr) and FP32 lists
+    std::vector<void*> bufs;           // prefix / kstart arrays of both
+    long long nq64 = 0, nq32 = 0;      // contracted quartets in each
+  } split;
 };
 
-// profile slots: ERI kernel of class c -> c; derivative part k of class c ->
-// QX_NUM_CLASSES_MAX + c * 12 + k
+// profile slots: ERI / J-K kernel of class c -> QX_SLOT_ERI(c, fp32);
+// derivative part k of class c -> QX_SLOT_DERIV(c, k, fp32)
 #define QX_NUM_CLASSES_MAX 21
-#define QX_PROFILE_SLOTS (QX_NUM_CLASSES_MAX * 13)
+#define QX_SLOT_ERI(c, f32) ((f32) * QX_NUM_CLASSES_MAX + (c))
+#define QX_SLOT_DERIV(c, k, f32) (2 * QX_NUM_CLASSES_MAX + ((f32) * QX_NUM_CLASSES_MAX + (c)) * QX_DPARTS_CAP + (k))
+#define QX_PROFILE_SLOTS (2 * QX_NUM_CLASSES_MAX * (1 + QX_DPARTS_CAP))
 
 // helper kernels (qx_kernels.hip)
 extern "C" {

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <numeric>
 #include <string>
@@ -211,7 +212,57 @@ int launch_eri(qx_ctx* c, const QxWorkDev& w, int cls, const QxEpiDev& e, int fp
   QxLaunch L{fp32, s};
   const QxClassEntry& ent = qx_class_table[cls];
   auto fn = e.epi == QX_EPI_JK ? ent.jk : ent.eri;
-  return timed_launch(c, cls, w.nquart, s, [&] { return fn(&a, &L); });
+  return timed_launch(c, QX_SLOT_ERI(cls, fp32 ? 1 : 0), w.nquart, s, [&] { return fn(&a, &L); });
+}
+
+// One kernel launch of a pass, with its estimated cost (flops).
+struct Launch {
+  double cost;
+  std::function<int(hipStream_t)> go;
+};
+
+void ensure_side_streams(qx_ctx* c) {
+  const int want = std::max(0, c->nstreams - 1);
+  while ((int)c->side.size() < want) {
+    hipStream_t t;
+    QX_HIP(hipStreamCreateWithFlags(&t, hipStreamNonBlocking));
+    c->side.push_back(t);
+  }
+  while ((int)c->fork_ev.size() < want + 1) {
+    hipEvent_t e;
+    QX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    c->fork_ev.push_back(e);
+  }
+}
+
+void check_launch(int rc, const char* what);
+
+// Runs the independent launches of one pass.  With several streams the
+// costliest go first, each to the least-loaded stream; the side streams wait
+// for the caller's stream before starting and the caller's stream waits for
+// all of them at the end, so the pass is ordered on `s` like one kernel.
+// With profiling on, each kernel's HIP events sit on the stream it runs on:
+// concurrent kernels are timed in place (one stream times each alone).
+void run_launches(qx_ctx* c, std::vector<Launch>& v, hipStream_t s, const char* what) {
+  const int ns = std::max(1, std::min<int>(c->nstreams, (int)v.size()));
+  if (ns == 1) {
+    for (auto& x : v) check_launch(x.go(s), what);
+    return;
+  }
+  ensure_side_streams(c);
+  std::stable_sort(v.begin(), v.end(), [](const Launch& a, const Launch& b) { return a.cost > b.cost; });
+  QX_HIP(hipEventRecord(c->fork_ev[0], s));
+  for (int k = 0; k < ns - 1; ++k) QX_HIP(hipStreamWaitEvent(c->side[k], c->fork_ev[0], 0));
+  std::vector<double> load(ns, 0.0);
+  for (auto& x : v) {
+    const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    load[k] += x.cost;
+    check_launch(x.go(k == 0 ? s : c->side[k - 1]), what);
+  }
+  for (int k = 0; k < ns - 1; ++k) {
+    QX_HIP(hipEventRecord(c->fork_ev[k + 1], c->side[k]));
+    QX_HIP(hipStreamWaitEvent(s, c->fork_ev[k + 1], 0));
+  }
 }
 
 void check_launch(int rc, const char* what) {
@@ -399,6 +450,90 @@ void schwarz_and_lists(qx_ctx* c) {
   for (auto& wh : c->work) wh.w.prefix = c->d_prefix + wh.prefix_off;
 }
 
+// Mixed precision: the canonical quartet lists split at a Schwarz-bound
+// threshold.  Kets of each bra pair are sorted by descending Q, so for bra i
+// the FP64 kets (Q_i Q_j >= thr^2) are a prefix of its admissible kets and
+// the FP32 kets the rest; the FP32 list starts each bra's kets at kstart.
+void build_split(qx_ctx* c, double thr) {
+  auto& S = c->split;
+  if (S.thr == thr) return;
+  QX_HIP(hipDeviceSynchronize());  // no kernel may still read the old lists
+  for (void* p : S.bufs) (void)hipFree(p);
+  S = qx_ctx::Split{};
+  const double tol2 = c->tol * c->tol, thr2 = thr * thr;
+  const int nclass = pair_class(QX_LMAX_SUPPORTED, QX_LMAX_SUPPORTED) + 1;
+  std::vector<long long> pre64, pre32;
+  std::vector<int> kst;
+  std::vector<long long> off64, off32, offk;
+  auto count_ge = [&](double Qi, int ko, int kn, double bound) {
+    int lo = 0, hi = kn;
+    while (lo < hi) {
+      const int mid = (lo + hi) / 2;
+      if (Qi * c->h_pair_Q[ko + mid] >= bound) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+  };
+  for (int X = 0; X < nclass; ++X)
+    for (int Y = 0; Y <= X; ++Y) {
+      const int bo = c->class_pair_off[X], bn = c->class_pair_cnt[X];
+      const int ko = c->class_pair_off[Y], kn = c->class_pair_cnt[Y];
+      if (bn == 0 || kn == 0) continue;
+      std::vector<long long> kcum(kn + 1, 0);
+      for (int j = 0; j < kn; ++j) kcum[j + 1] = kcum[j] + c->h_pp_n[ko + j];
+      QxWorkHost w{};
+      w.cls = class_index(X, Y);
+      w.bra_class = X; w.ket_class = Y;
+      const int u0 = c->h_pp_first[bo];
+      const int u1 = c->h_pp_first[bo + bn - 1] + c->h_pp_n[bo + bn - 1];
+      w.w.bra_off = u0; w.w.nbra = u1 - u0;
+      w.w.ket_off = c->h_pp_first[ko];
+      w.w.nket = (int)kcum[kn];
+      w.w.diag = 0;
+      QxWorkHost w32 = w;
+      off64.push_back((long long)pre64.size());
+      off32.push_back((long long)pre32.size());
+      offk.push_back((long long)kst.size());
+      pre64.push_back(0);
+      pre32.push_back(0);
+      long long run64 = 0, run32 = 0;
+      for (int i = 0; i < bn; ++i) {
+        const double Qi = c->h_pair_Q[bo + i];
+        int kc = count_ge(Qi, ko, kn, tol2);
+        if (X == Y) kc = std::min(kc, i + 1);
+        const int k64 = std::min(kc, count_ge(Qi, ko, kn, thr2));
+        S.nq64 += k64;
+        S.nq32 += kc - k64;
+        for (int k = 0; k < c->h_pp_n[bo + i]; ++k) {
+          run64 += kcum[k64];
+          run32 += kcum[kc] - kcum[k64];
+          pre64.push_back(run64);
+          pre32.push_back(run32);
+          kst.push_back((int)kcum[k64]);
+        }
+      }
+      w.w.nquart = run64;
+      w32.w.nquart = run32;
+      S.w64.push_back(w);
+      S.w32.push_back(w32);
+    }
+  auto up = [&](const void* src, size_t bytes) {
+    void* d = nullptr;
+    QX_HIP(hipMalloc(&d, std::max<size_t>(bytes, 8)));
+    S.bufs.push_back(d);
+    if (bytes) QX_HIP(hipMemcpy(d, src, bytes, hipMemcpyHostToDevice));
+    return d;
+  };
+  const long long* d64 = static_cast<const long long*>(up(pre64.data(), pre64.size() * sizeof(long long)));
+  const long long* d32 = static_cast<const long long*>(up(pre32.data(), pre32.size() * sizeof(long long)));
+  const int* dk = static_cast<const int*>(up(kst.data(), kst.size() * sizeof(int)));
+  for (size_t i = 0; i < S.w64.size(); ++i) {
+    S.w64[i].w.prefix = d64 + off64[i];
+    S.w32[i].w.prefix = d32 + off32[i];
+    S.w32[i].w.kstart = dk + offk[i];
+  }
+  S.thr = thr;
+}
+
 void setup_shell_tables(qx_ctx* c) {
   QxShellsDev& s = c->shells;
   s.nshell = c->nshell; s.natom = c->natom; s.ncart = c->ncart; s.nbf = c->nbf; s.pure = c->pure;
@@ -516,6 +651,10 @@ int qx_finalize(qx_ctx* c) {
     (void)hipEventDestroy(p.stop);
   }
   for (auto e : c->event_pool) (void)hipEventDestroy(e);
+  (void)hipDeviceSynchronize();
+  for (auto t : c->side) (void)hipStreamDestroy(t);
+  for (auto e : c->fork_ev) (void)hipEventDestroy(e);
+  for (void* p : c->split.bufs) (void)hipFree(p);
   for (void* p : c->allocs) (void)hipFree(p);
   delete c;
   return QX_OK;
@@ -609,18 +748,43 @@ int qx_compute_2e_int(qx_ctx* c, double* out, void* stream) {
   return QX_OK;
 }
 
-static void launch_deriv_all(qx_ctx* c, const QxEpiDev& e, int fp32, hipStream_t s) {
-  for (auto& wh : c->work) {
+// Derivative kernels (every part of every class) over one set of lists.
+static void add_deriv_launches(qx_ctx* c, const std::vector<QxWorkHost>& work, const QxEpiDev& e, int fp32,
+                               std::vector<Launch>& out) {
+  for (const auto& wh : work) {
+    if (wh.w.nquart <= 0) continue;
     const QxClassEntry& ent = qx_class_table[wh.cls];
     for (int k = 0; k < ent.nparts; ++k) {
       QxArgs a;
       a.pr = c->pr; a.pp = c->pp; a.w = wh.w; a.e = e; a.boys = c->d_boys;
-      QxLaunch L{fp32, s};
-      const int slot = QX_NUM_CLASSES + wh.cls * 12 + k;
+      const int slot = QX_SLOT_DERIV(wh.cls, k, fp32 ? 1 : 0);
       auto fn = e.epi == QX_EPI_GRAD ? ent.grad[k] : ent.deri[k];
-      check_launch(timed_launch(c, slot, wh.w.nquart, s, [&] { return fn(&a, &L); }), "eri derivative");
+      const long long nq = wh.w.nquart;
+      out.push_back({(double)nq * ent.part_flops[k], [c, a, fn, fp32, slot, nq](hipStream_t st) {
+                       QxLaunch L{fp32, st};
+                       return timed_launch(c, slot, nq, st, [&] { return fn(&a, &L); });
+                     }});
     }
   }
+}
+
+// J/K kernels of every class over one set of lists.
+static void add_jk_launches(qx_ctx* c, const std::vector<QxWorkHost>& work, const QxEpiDev& e, int fp32,
+                            std::vector<Launch>& out) {
+  for (const auto& wh : work) {
+    if (wh.w.nquart <= 0) continue;
+    const QxWorkDev w = wh.w;
+    const int cls = wh.cls;
+    out.push_back({(double)w.nquart * qx_class_table[cls].eri_flops, [c, w, cls, e, fp32](hipStream_t st) {
+                     return launch_eri(c, w, cls, e, fp32, st, c->pr, c->pp);
+                   }});
+  }
+}
+
+static void launch_deriv_all(qx_ctx* c, const QxEpiDev& e, int fp32, hipStream_t s) {
+  std::vector<Launch> v;
+  add_deriv_launches(c, c->work, e, fp32, v);
+  run_launches(c, v, s, "eri derivative");
 }
 
 static void deriv_tensor(qx_ctx* c, int coord, double* out, hipStream_t s) {
@@ -685,6 +849,7 @@ int qx_compute_2e_deriv(qx_ctx* c, const int* dv, int len, double* out, void* st
 int qx_jk_build(qx_ctx* c, const double* D, double* J, double* K, int precision, double thr, void* stream) {
   if (!c || !D || !J || !K) return QX_EINVAL;
   if (precision != QX_PREC_FP64 && precision != QX_PREC_MIXED) return QX_EINVAL;
+  if (precision == QX_PREC_MIXED && !(thr >= 0.0 && std::isfinite(thr))) return QX_EINVAL;
   try {
     QX_HIP(hipSetDevice(c->device));
     hipStream_t s = (hipStream_t)stream;
@@ -700,11 +865,15 @@ int qx_jk_build(qx_ctx* c, const double* D, double* J, double* K, int precision,
     check_launch(qxk_fill(Kp.d(), (long long)(nc * nc), 0.0, s), "fill");
     QxEpiDev e{};
     e.epi = QX_EPI_JK; e.nbf = (int)nc; e.natom = c->natom; e.coord = -1; e.D = Dcart; e.J = Jp.d(); e.K = Kp.d();
+    std::vector<Launch> v;
     if (precision == QX_PREC_MIXED) {
-      c->err = "mixed precision J/K is not built";
-      return QX_EUNSUPPORTED;
+      build_split(c, thr);
+      add_jk_launches(c, c->split.w64, e, 0, v);
+      add_jk_launches(c, c->split.w32, e, 1, v);
+    } else {
+      add_jk_launches(c, c->work, e, 0, v);
     }
-    for (auto& wh : c->work) check_launch(launch_eri(c, wh.w, wh.cls, e, 0, s, c->pr, c->pp), "jk_build");
+    run_launches(c, v, s, "jk_build");
     if (c->pure) {
       DevBuf Js(nc * nc * sizeof(double)), Ks(nc * nc * sizeof(double));
       check_launch(qxk_sym_half(Jp.d(), Js.d(), (int)nc, s), "sym");
@@ -726,6 +895,7 @@ int qx_jk_build(qx_ctx* c, const double* D, double* J, double* K, int precision,
 int qx_eri_grad(qx_ctx* c, const double* D, double* grad, int precision, double thr, void* stream) {
   if (!c || !D || !grad) return QX_EINVAL;
   if (precision != QX_PREC_FP64 && precision != QX_PREC_MIXED) return QX_EINVAL;
+  if (precision == QX_PREC_MIXED && !(thr >= 0.0 && std::isfinite(thr))) return QX_EINVAL;
   try {
     QX_HIP(hipSetDevice(c->device));
     hipStream_t s = (hipStream_t)stream;
@@ -736,13 +906,17 @@ int qx_eri_grad(qx_ctx* c, const double* D, double* grad, int precision, double 
       check_launch(qxk_ao_to_cart_mat(&c->shells, D, Dc->d(), s), "ao_to_cart");
       Dcart = Dc->d();
     }
-    if (precision == QX_PREC_MIXED) {
-      c->err = "mixed precision gradient is not built";
-      return QX_EUNSUPPORTED;
-    }
     QxEpiDev e{};
     e.epi = QX_EPI_GRAD; e.nbf = (int)nc; e.natom = c->natom; e.coord = -1; e.D = Dcart; e.grad = grad;
-    launch_deriv_all(c, e, 0, s);
+    std::vector<Launch> v;
+    if (precision == QX_PREC_MIXED) {
+      build_split(c, thr);
+      add_deriv_launches(c, c->split.w64, e, 0, v);
+      add_deriv_launches(c, c->split.w32, e, 1, v);
+    } else {
+      add_deriv_launches(c, c->work, e, 0, v);
+    }
+    run_launches(c, v, s, "eri_grad");
     QX_HIP(hipStreamSynchronize(s));
   } catch (const QxError& e) {
     return fail(c, e);
@@ -840,6 +1014,43 @@ int qx_eri_store(qx_ctx* c, double* out, void* stream) {
 int qx_num_classes(void) { return QX_NUM_CLASSES; }
 
 int qx_profile_slots(void) { return QX_PROFILE_SLOTS; }
+
+int qx_profile_slot(int slot, int* kind, int* cls, int* part, int* fp32) {
+  if (slot < 0 || slot >= QX_PROFILE_SLOTS) return QX_EINVAL;
+  int k, c, p = 0, f;
+  if (slot < 2 * QX_NUM_CLASSES_MAX) {
+    k = 0; f = slot / QX_NUM_CLASSES_MAX; c = slot % QX_NUM_CLASSES_MAX;
+  } else {
+    const int r = slot - 2 * QX_NUM_CLASSES_MAX;
+    k = 1; p = r % QX_DPARTS_CAP;
+    const int fc = r / QX_DPARTS_CAP;
+    f = fc / QX_NUM_CLASSES_MAX; c = fc % QX_NUM_CLASSES_MAX;
+  }
+  if (kind) *kind = k;
+  if (cls) *cls = c;
+  if (part) *part = p;
+  if (fp32) *fp32 = f;
+  return QX_OK;
+}
+
+int qx_mixed_quartets(qx_ctx* c, double thr, long long* n64, long long* n32) {
+  if (!c || !(thr >= 0.0 && std::isfinite(thr))) return QX_EINVAL;
+  try {
+    QX_HIP(hipSetDevice(c->device));
+    build_split(c, thr);
+    if (n64) *n64 = c->split.nq64;
+    if (n32) *n32 = c->split.nq32;
+  } catch (const QxError& e) {
+    return fail(c, e);
+  }
+  return QX_OK;
+}
+
+int qx_set_streams(qx_ctx* c, int n) {
+  if (!c || n < 1 || n > 8) return QX_EINVAL;
+  c->nstreams = n;
+  return QX_OK;
+}
 
 int qx_class_info(int cls, int* l4, int* nparts) {
   if (cls < 0 || cls >= QX_NUM_CLASSES) return QX_EINVAL;

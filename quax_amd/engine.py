@@ -114,6 +114,16 @@ class EriEngine:
             out.append((tuple(l4), eri[k], deri[k], npart.value))
         return out
 
+    def mixed_quartets(self, fp32_threshold: float):
+        """(FP64, FP32) contracted quartet counts of the mixed-precision split."""
+        n64, n32 = ctypes.c_longlong(), ctypes.c_longlong()
+        check(self.L.qx_mixed_quartets(self.ctx, float(fp32_threshold), ctypes.byref(n64), ctypes.byref(n32)),
+              self.ctx, "mixed_quartets")
+        return n64.value, n32.value
+
+    def set_streams(self, n: int):
+        check(self.L.qx_set_streams(self.ctx, int(n)), self.ctx, "set_streams")
+
     def profile(self, on: bool):
         check(self.L.qx_profile_enable(self.ctx, 1 if on else 0), self.ctx)
 

@@ -64,6 +64,14 @@ int main(int argc, char** argv) {
   if ((rc = qx_oei(c, S.data(), T.data(), V.data(), nullptr))) die(c, rc, "oei");
   if ((rc = qx_oei_grad(c, D.data(), D.data(), g1.data(), nullptr))) die(c, rc, "oei_grad");
   for (auto* v : {&J, &K, &g, &S, &T, &V, &g1}) std::fwrite(v->data(), 8, v->size(), o);
+  // mixed precision: every quartet in FP32 (threshold above any bound), then
+  // a split at 0.05 (both kernel variants in one pass)
+  for (double thr : {1e300, 0.05}) {
+    std::vector<double> Jm(n2), Km(n2), gm(3 * natom, 0.0);
+    if ((rc = qx_jk_build(c, D.data(), Jm.data(), Km.data(), 1, thr, nullptr))) die(c, rc, "jk_build mixed");
+    if ((rc = qx_eri_grad(c, D.data(), gm.data(), 1, thr, nullptr))) die(c, rc, "eri_grad mixed");
+    for (auto* v : {&Jm, &Km, &gm}) std::fwrite(v->data(), 8, v->size(), o);
+  }
   std::fclose(o);
   qx_finalize(c);
   std::fprintf(stderr, "emu: ok\n");

@@ -77,6 +77,18 @@ inline hipError_t hipEventDestroy(hipEvent_t e) {
   std::free(e);
   return hipSuccess;
 }
+inline hipError_t hipEventCreateWithFlags(hipEvent_t* e, unsigned) { return hipEventCreate(e); }
+enum { hipStreamNonBlocking = 1, hipEventDisableTiming = 2 };
+// kernels run synchronously at launch, so streams carry no ordering
+inline hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned) {
+  *s = std::malloc(8);
+  return hipSuccess;
+}
+inline hipError_t hipStreamDestroy(hipStream_t s) {
+  std::free(s);
+  return hipSuccess;
+}
+inline hipError_t hipStreamWaitEvent(hipStream_t, hipEvent_t, unsigned) { return hipSuccess; }
 inline hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
 inline hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
 inline hipError_t hipEventElapsedTime(float* t, hipEvent_t, hipEvent_t) {

@@ -74,7 +74,8 @@ def run(name, tol=1e-14, seed=3):
     if nbf <= 40:
         res["dG"] = out[pos:pos + 3 * N * n4].reshape(3 * N, nbf, nbf, nbf, nbf)
         pos += 3 * N * n4
-    for k, sz in (("J", n2), ("K", n2), ("g", 3 * N), ("S", n2), ("T", n2), ("V", n2), ("g1", 3 * N)):
+    for k, sz in (("J", n2), ("K", n2), ("g", 3 * N), ("S", n2), ("T", n2), ("V", n2), ("g1", 3 * N),
+                  ("J32", n2), ("K32", n2), ("g32", 3 * N), ("Jmx", n2), ("Kmx", n2), ("gmx", 3 * N)):
         res[k] = out[pos:pos + sz]
         pos += sz
     assert pos == out.size
@@ -103,6 +104,10 @@ def run(name, tol=1e-14, seed=3):
     dS, dT, dV = ob.oei_deriv1()
     g1_ref = np.einsum("xpq,pq->x", dT + dV, Dc) + np.einsum("xpq,pq->x", dS, Dc)
     errs["g1"] = np.abs(res["g1"] - g1_ref).max()
+    # mixed precision against the FP64 path of the same build
+    for k in ("J", "K", "g"):
+        errs[k + "32"] = np.abs(res[k + "32"] - res[k]).max()
+        errs[k + "mx"] = np.abs(res[k + "mx"] - res[k]).max()
     return errs
 
 

@@ -86,14 +86,46 @@ def test_class_table_static_info():
         assert L.qx_class_info(k, l4, ctypes.byref(npart)) == 0
         la, lb, lc, ld = l4[:]
         assert la >= lb and lc >= ld and (la * (la + 1) // 2 + lb) >= (lc * (lc + 1) // 2 + ld)
-        assert 1 <= npart.value <= 12
+        assert 1 <= npart.value <= 48
         seen.add((la, lb, lc, ld))
         ef = ctypes.c_double()
-        pf = (ctypes.c_double * 12)()
-        pc = (ctypes.c_int * 12)()
+        pf = (ctypes.c_double * 48)()
+        pc = (ctypes.c_int * 48)()
         assert L.qx_class_flops(k, ctypes.byref(ef), pf, pc) == 0
         assert ef.value > 0 and all(pf[i] > 0 for i in range(npart.value))
     assert len(seen) == 21
+
+
+def test_profile_slots_are_distinct():
+    from quax_amd import _lib
+    L = _lib.load()
+    seen = set()
+    for slot in range(L.qx_profile_slots()):
+        v = [ctypes.c_int() for _ in range(4)]
+        assert L.qx_profile_slot(slot, *[ctypes.byref(x) for x in v]) == 0
+        key = tuple(x.value for x in v)
+        assert key not in seen and key[0] in (0, 1) and key[3] in (0, 1) and 0 <= key[1] < 21
+        seen.add(key)
+    assert L.qx_profile_slot(L.qx_profile_slots(), None, None, None, None) != 0
+
+
+def test_derivative_parts_partition_the_outputs():
+    """Every class's derivative parts cover each (a, b) output pair once and
+    stay within the node budget unless already split to single (a, b)."""
+    import sys
+    sys.path.insert(0, os.path.join(ROOT, "quax_amd", "csrc"))
+    import gen_eri as G
+    _, classes = G.canonical_classes(2)
+    for cls in classes:
+        parts = G.deriv_parts(cls)
+        na, nb = G.ncart(cls[0]), G.ncart(cls[1])
+        cover = np.zeros((na, nb), dtype=int)
+        for p in parts:
+            (a0, a1), (b0, b1) = p["a"], p["b"]
+            cover[a0:a1, b0:b1] += 1
+            assert p["dinv"] and len(p["combos"]) == 9
+        assert (cover == 1).all(), cls
+        assert len(parts) <= 48
 
 
 def test_sph_transform_orthonormal():
