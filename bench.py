@@ -65,14 +65,26 @@ def kernel_table(eng):
         nm = "".join(map(str, l4))
         tp = "float" if f32.value else "double"
         N = ncart(l4[0]) * ncart(l4[1]) * ncart(l4[2]) * ncart(l4[3])
+        key = ("jk" if kind.value == 0 else "deriv") + ("32" if f32.value else "64")
         if kind.value == 0:
             # J/K epilogue: 6 fused multiply-adds per integral
-            info[slot] = (f"qx_kj_e{nm}<{tp}>", ef + 12.0 * N)
+            info[slot] = (f"qx_kj_e{nm}<{tp}>", ef + 12.0 * N, nm, key)
         elif part.value < npart:
-            info[slot] = (f"qx_kg_d{nm}_{part.value}<{tp}>", pf[part.value])
+            info[slot] = (f"qx_kg_d{nm}_{part.value}<{tp}>", pf[part.value], nm, key)
         else:
-            info[slot] = (f"unused slot {slot}", 0.0)
+            info[slot] = (f"unused slot {slot}", 0.0, nm, key)
     return info
+
+
+def class_times(table, ms, steps):
+    """Kernel ms per step summed per class: {class: {jk64, deriv64, ...}}."""
+    out = {}
+    for k, t in enumerate(ms):
+        if t > 0:
+            _, _, nm, key = table[k]
+            out.setdefault(nm, {})
+            out[nm][key] = out[nm].get(key, 0.0) + float(t) / steps
+    return out
 
 
 def cpu_baseline(mol, basis, tol, eng, seconds):
@@ -207,6 +219,11 @@ def main():
     ap.add_argument("--streams", type=int, default=4, help="streams per pass (1 = every kernel timed alone)")
     ap.add_argument("--no-scf", action="store_true",
                     help="skip the SCF and use the core-Hamiltonian density (profiling runs)")
+    ap.add_argument("--sweep", default="",
+                    help="comma-separated FP32 thresholds: mixed HF energy + gradient error and time at each")
+    ap.add_argument("--no-mixed", action="store_true", help="skip every mixed-precision run (FP64-only builds)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run a plain 512 MiB device copy (HBM-counter calibration for profiler runs)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is design-only)",
@@ -239,9 +256,23 @@ def main():
     cq = eng.class_quartets()
     q_eri = int(sum(x[1] for x in cq))
     q_deri = int(sum(x[2] for x in cq))
-    n64, n32 = eng.mixed_quartets(thr)
+    n64, n32 = eng.mixed_quartets(thr) if not args.no_mixed else (q_eri, 0)
     log(f"[bench] {args.workload}: nbf {eng.nbf} (cart {eng.ncart}), ERI quartets {q_eri}, dERI quartets {q_deri}, "
         f"init {t_init:.2f} s; mixed @ {thr:g}: {n32 / max(n64 + n32, 1):.1%} of quartets in FP32")
+
+    if args.copy_probe:
+        src = torch.ones(64 << 20, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+    # first-launch costs (code-object loading, side streams) stay out of the timings
+    S0, T0, V0 = eng.oei()
+    for prec in ((0,) if args.no_mixed else (0, 1)):
+        eng.jk(S0, precision=prec, fp32_threshold=thr)
+        eng.eri_grad(S0, precision=prec, fp32_threshold=thr)
+    torch.cuda.synchronize()
 
     hf = {"fp32_threshold": thr, "fp32_quartet_fraction": n32 / max(n64 + n32, 1)}
     if args.no_scf:
@@ -254,29 +285,48 @@ def main():
         E, aux, grad, t_scf, t_grad = hf_energy_gradient(eng, mol, "fp64", thr)
         log(f"[bench] FP64 RHF E = {E:.10f} in {aux['iterations']} iterations ({t_scf:.2f} s), "
             f"gradient {t_grad:.3f} s, |sum g| = {abs(grad.reshape(-1, 3).sum(0)).max():.2e}")
+        if args.no_mixed:
+            raise SystemExit("--no-mixed needs --no-scf")
         Em, auxm, gradm, t_scfm, t_gradm = hf_energy_gradient(eng, mol, "mixed", thr)
         log(f"[bench] mixed RHF E = {Em:.10f} (error {Em - E:.2e}) in {auxm['iterations']} iterations "
             f"({t_scfm:.2f} s), gradient {t_gradm:.3f} s, max gradient error {np.abs(gradm - grad).max():.2e}")
         D = aux["D"].contiguous()
+        sweep = []
+        for t in (float(x) for x in args.sweep.split(",") if x.strip()):
+            n64t, n32t = eng.mixed_quartets(t)
+            Et, auxt, gt, ts, tg = hf_energy_gradient(eng, mol, "mixed", t)
+            sweep.append({"fp32_threshold": t, "fp32_quartet_fraction": n32t / max(n64t + n32t, 1),
+                          "energy_error": Et - E, "grad_max_error": float(np.abs(gt - grad).max()),
+                          "scf_iterations": auxt["iterations"], "wall_s": ts + tg})
+            log(f"[bench] sweep {t:g}: FP32 {sweep[-1]['fp32_quartet_fraction']:.1%}, dE {Et - E:.2e}, "
+                f"max dg {sweep[-1]['grad_max_error']:.2e}, {ts + tg:.2f} s")
+        if sweep:
+            hf["sweep"] = sweep
         hf.update({"energy": E, "scf_iterations": aux["iterations"], "scf_wall_s": t_scf, "grad_wall_s": t_grad,
                    "hf_gradient_wall_s_fp64": t_scf + t_grad,
                    "hf_gradient_wall_s_mixed": t_scfm + t_gradm, "scf_iterations_mixed": auxm["iterations"],
                    "energy_error_mixed": Em - E, "grad_max_error_mixed": float(np.abs(gradm - grad).max()),
                    "speedup_mixed": (t_scf + t_grad) / (t_scfm + t_gradm)})
 
-    # headline: FP64 steps (J/K pass + adjoint gradient pass), kernels timed with HIP events
-    dt, (ms, launches, pq) = timed_steps(eng, D, args.steps, args.warmup, 0, 0.0, True)
+    # headline: FP64 steps (J/K pass + adjoint gradient pass) on args.streams streams
+    dt, _ = timed_steps(eng, D, args.steps, args.warmup, 0, 0.0, False)
     ms_step = dt * 1e3 / args.steps
     value = (q_eri + q_deri) / (dt / args.steps)
-    dtm, _ = timed_steps(eng, D, args.steps, 1, 1, thr, False)
-    ms_step_mixed = dtm * 1e3 / args.steps
+    dtm = None
+    if not args.no_mixed:
+        dtm, _ = timed_steps(eng, D, args.steps, 1, 1, thr, False)
+    # roofline pass: the same steps on one stream, every kernel timed alone by
+    # HIP events on its launch stream
+    eng.set_streams(1)
+    dts, (ms, launches, pq) = timed_steps(eng, D, args.steps, 0, 0, 0.0, True)
+    eng.set_streams(args.streams)
 
     table = kernel_table(eng)
     eri_slots = [k for k in range(len(ms)) if table[k][0].startswith("qx_kj")]
     eri_ms = float(ms[eri_slots].sum()) / args.steps
     deri_ms = float(ms.sum()) / args.steps - eri_ms
     slot = int(np.argmax(ms))
-    name, fl = table[slot]
+    name, fl = table[slot][:2]
     avg_ms = ms[slot] / launches[slot]
     flops_per_launch = fl * pq[slot] / launches[slot]
     achieved = flops_per_launch / (avg_ms * 1e-3)
@@ -316,16 +366,19 @@ def main():
                                f"{'spherical' if pure else 'Cartesian'} basis ({eng.nbf} AO / {eng.ncart} Cartesian), "
                                f"Schwarz 1e-12; step = direct J/K ERI pass + adjoint dERI gradient pass",
                    "eri_quartets": q_eri, "deri_quartets": q_deri, "streams": args.streams,
+                   "one_stream_ms_per_step": dts * 1e3 / args.steps,
                    "eri_kernel_ms": eri_ms, "deri_kernel_ms": deri_ms},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64, "traffic": None, "kernel": name, "avg_launch_ms": avg_ms,
                      "flops_per_launch": flops_per_launch, "step_flops": total_flops,
                      "step_tflops": total_flops / (ms_step * 1e-3) / 1e12},
         "cpu_baseline": cpu,
-        "mixed": {"fp32_threshold": thr, "ms_per_step": ms_step_mixed,
-                  "value": (q_eri + q_deri) / (dtm / args.steps), "speedup_vs_fp64_step": ms_step / ms_step_mixed},
+        "mixed": None if dtm is None else
+        {"fp32_threshold": thr, "ms_per_step": dtm * 1e3 / args.steps,
+         "value": (q_eri + q_deri) / (dtm / args.steps), "speedup_vs_fp64_step": dt / dtm},
         "hf": hf,
         "top_kernels_ms_per_step": kernels,
+        "class_kernel_ms_per_step": class_times(table, ms, args.steps),
     }
     print(json.dumps(out), flush=True)
     eng.close()

@@ -9,7 +9,8 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "lib", "libquax_hip.so")
+# QUAX_AMD_LIB selects another build of the library (e.g. a tuning variant)
+LIB_PATH = os.environ.get("QUAX_AMD_LIB") or os.path.join(_HERE, "lib", "libquax_hip.so")
 
 QX_OK, QX_EINVAL, QX_ENOMEM, QX_EHIP, QX_EUNSUPPORTED, QX_ENODEVICE = range(6)
 QX_PREC_FP64, QX_PREC_MIXED = 0, 1

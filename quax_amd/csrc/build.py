@@ -5,7 +5,7 @@ compile every translation unit with hipcc in parallel, link the shared
 library into quax_amd/lib/.  Objects are cached under csrc/_build keyed by a
 hash of (source, headers, flags); ``force=True`` recompiles everything.
 
-Usage: python -m quax_amd.csrc.build [--force] [--jobs N] [--lmax L] [--no-fp32]
+Usage: python -m quax_amd.csrc.build [--force] [--jobs N] [--lmax L] [--no-fp32] [--budget N] [--out PATH]
 """
 from __future__ import annotations
 
@@ -66,13 +66,20 @@ def _compile(src, force):
     return obj, time.time() - t0, False
 
 
-def build(force=False, jobs=None, lmax=2, fp32=True, verbose=True):
+def build(force=False, jobs=None, lmax=2, fp32=True, verbose=True, budget=None, out=None):
+    """budget: one derivative node budget for every class instead of the
+    tuned per-class table (tuning builds); out: library path."""
+    lib = out or LIB
     os.makedirs(OBJ, exist_ok=True)
-    os.makedirs(LIBDIR, exist_ok=True)
+    os.makedirs(os.path.dirname(os.path.abspath(lib)), exist_ok=True)
     gen_args = [sys.executable, os.path.join(HERE, "gen_eri.py"), GEN, str(lmax)] + ([] if fp32 else ["--no-fp32"])
+    env = dict(os.environ)
+    env.pop("QX_DERIV_NODES", None)
+    if budget:
+        env["QX_DERIV_NODES"] = str(int(budget))
     if os.path.isdir(GEN):
         shutil.rmtree(GEN)
-    subprocess.run(gen_args, check=True, capture_output=True)
+    subprocess.run(gen_args, check=True, capture_output=True, env=env)
     gen_srcs = sorted(os.path.join(GEN, f) for f in os.listdir(GEN) if f.endswith(".hip"))
     # biggest first so the long poles start early
     gen_srcs.sort(key=lambda p: -os.path.getsize(p))
@@ -87,12 +94,12 @@ def build(force=False, jobs=None, lmax=2, fp32=True, verbose=True):
             objs.append(obj)
             if verbose and not cached:
                 print(f"  compiled {os.path.basename(futs[f])} in {dt:.1f} s", flush=True)
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", LIB + ".tmp"] + sorted(objs)
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", lib + ".tmp"] + sorted(objs)
     subprocess.run(cmd, check=True)
-    os.replace(LIB + ".tmp", LIB)
+    os.replace(lib + ".tmp", lib)
     if verbose:
-        print(f"built {LIB} ({len(objs)} objects, {time.time() - t0:.1f} s)", flush=True)
-    return LIB
+        print(f"built {lib} ({len(objs)} objects, {time.time() - t0:.1f} s)", flush=True)
+    return lib
 
 
 if __name__ == "__main__":
@@ -101,5 +108,8 @@ if __name__ == "__main__":
     ap.add_argument("--jobs", type=int, default=None)
     ap.add_argument("--lmax", type=int, default=2)
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 (mixed-precision) kernels")
+    ap.add_argument("--budget", type=int, default=None,
+                    help="one derivative node budget for all classes (tuning builds)")
+    ap.add_argument("--out", default=None, help="library path (default quax_amd/lib/libquax_hip.so)")
     a = ap.parse_args()
-    build(force=a.force, jobs=a.jobs, lmax=a.lmax, fp32=not a.no_fp32)
+    build(force=a.force, jobs=a.jobs, lmax=a.lmax, fp32=not a.no_fp32, budget=a.budget, out=a.out)

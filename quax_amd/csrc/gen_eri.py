@@ -28,12 +28,21 @@ Usage: python gen_eri.py OUTDIR [LMAX] [--no-fp32]
 """
 from __future__ import annotations
 
+import json
 import os
 import sys
 
 AX = "xyz"
-# largest SSA node count allowed in one derivative body before it is split
-DERIV_NODES = int(os.environ.get("QX_DERIV_NODES", "2000"))
+# largest SSA node count allowed in one derivative body before it is split:
+# a per-class table (picked by timing budget variants on the GPU), else
+# DERIV_NODES; QX_DERIV_NODES in the environment overrides both (tuning builds)
+DERIV_NODES = 2000
+_BUDGET_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "deriv_budget.json")
+DERIV_BUDGET = {}
+if os.path.exists(_BUDGET_FILE):  # written by tune.py from GPU timings
+    with open(_BUDGET_FILE) as _fh:
+        DERIV_BUDGET = {tuple(int(ch) for ch in k): int(v) for k, v in json.load(_fh)["budget"].items()}
+_BUDGET_OVERRIDE = os.environ.get("QX_DERIV_NODES")
 
 
 def comps(l):
@@ -235,7 +244,8 @@ def deriv_parts(l, budget=None):
     recurrence nodes: whole class, else contiguous ranges of a components,
     else one a component with ranges of b components (one (a, b) pair per
     part is the finest split)."""
-    budget = budget or DERIV_NODES
+    if budget is None:
+        budget = int(_BUDGET_OVERRIDE) if _BUDGET_OVERRIDE else DERIV_BUDGET.get(tuple(l), DERIV_NODES)
     la, lb = l[0], l[1]
     NA, NB = ncart(la), ncart(lb)
     combos = [(c, x) for c in "ABC" for x in range(3)]

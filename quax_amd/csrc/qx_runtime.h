@@ -87,6 +87,13 @@ struct qx_ctx {
   std::vector<hipStream_t> side;
   std::vector<hipEvent_t> fork_ev;     // [0] fork, [1..] joins
 
+  // persistent n_cart^2 work matrices of the J/K and gradient passes (J', K',
+  // Cartesian D, symmetrised J and K); ws_ev orders a pass after the previous
+  // pass that used them, whatever stream each ran on
+  double* ws = nullptr;
+  hipEvent_t ws_ev = nullptr;
+  bool ws_used = false;
+
   // mixed precision: quartet lists split at one Schwarz-bound threshold
   struct Split {
     double thr = -1.0;

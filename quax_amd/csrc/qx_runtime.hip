@@ -450,6 +450,21 @@ void schwarz_and_lists(qx_ctx* c) {
   for (auto& wh : c->work) wh.w.prefix = c->d_prefix + wh.prefix_off;
 }
 
+// Work matrices of a J/K or gradient pass on stream s (see qx_ctx::ws).
+double* begin_ws(qx_ctx* c, hipStream_t s) {
+  if (!c->ws) {
+    c->ws = dalloc<double>(c, (size_t)5 * c->ncart * c->ncart);
+    QX_HIP(hipEventCreateWithFlags(&c->ws_ev, hipEventDisableTiming));
+  }
+  if (c->ws_used) QX_HIP(hipStreamWaitEvent(s, c->ws_ev, 0));
+  return c->ws;
+}
+
+void end_ws(qx_ctx* c, hipStream_t s) {
+  QX_HIP(hipEventRecord(c->ws_ev, s));
+  c->ws_used = true;
+}
+
 // Mixed precision: the canonical quartet lists split at a Schwarz-bound
 // threshold.  Kets of each bra pair are sorted by descending Q, so for bra i
 // the FP64 kets (Q_i Q_j >= thr^2) are a prefix of its admissible kets and
@@ -654,6 +669,7 @@ int qx_finalize(qx_ctx* c) {
   (void)hipDeviceSynchronize();
   for (auto t : c->side) (void)hipStreamDestroy(t);
   for (auto e : c->fork_ev) (void)hipEventDestroy(e);
+  if (c->ws_ev) (void)hipEventDestroy(c->ws_ev);
   for (void* p : c->split.bufs) (void)hipFree(p);
   for (void* p : c->allocs) (void)hipFree(p);
   delete c;
@@ -853,18 +869,18 @@ int qx_jk_build(qx_ctx* c, const double* D, double* J, double* K, int precision,
   try {
     QX_HIP(hipSetDevice(c->device));
     hipStream_t s = (hipStream_t)stream;
-    const size_t nc = c->ncart;
-    DevBuf Jp(nc * nc * sizeof(double)), Kp(nc * nc * sizeof(double));
-    std::unique_ptr<DevBuf> Dc(c->pure ? new DevBuf(nc * nc * sizeof(double)) : nullptr);
+    const size_t nc = c->ncart, n2 = nc * nc;
+    double* w = begin_ws(c, s);
+    double *Jp = w, *Kp = w + n2, *Dc = w + 2 * n2, *Js = w + 3 * n2, *Ks = w + 4 * n2;
     const double* Dcart = D;
     if (c->pure) {
-      check_launch(qxk_ao_to_cart_mat(&c->shells, D, Dc->d(), s), "ao_to_cart");
-      Dcart = Dc->d();
+      check_launch(qxk_ao_to_cart_mat(&c->shells, D, Dc, s), "ao_to_cart");
+      Dcart = Dc;
     }
-    check_launch(qxk_fill(Jp.d(), (long long)(nc * nc), 0.0, s), "fill");
-    check_launch(qxk_fill(Kp.d(), (long long)(nc * nc), 0.0, s), "fill");
+    check_launch(qxk_fill(Jp, (long long)n2, 0.0, s), "fill");
+    check_launch(qxk_fill(Kp, (long long)n2, 0.0, s), "fill");
     QxEpiDev e{};
-    e.epi = QX_EPI_JK; e.nbf = (int)nc; e.natom = c->natom; e.coord = -1; e.D = Dcart; e.J = Jp.d(); e.K = Kp.d();
+    e.epi = QX_EPI_JK; e.nbf = (int)nc; e.natom = c->natom; e.coord = -1; e.D = Dcart; e.J = Jp; e.K = Kp;
     std::vector<Launch> v;
     if (precision == QX_PREC_MIXED) {
       build_split(c, thr);
@@ -875,17 +891,15 @@ int qx_jk_build(qx_ctx* c, const double* D, double* J, double* K, int precision,
     }
     run_launches(c, v, s, "jk_build");
     if (c->pure) {
-      DevBuf Js(nc * nc * sizeof(double)), Ks(nc * nc * sizeof(double));
-      check_launch(qxk_sym_half(Jp.d(), Js.d(), (int)nc, s), "sym");
-      check_launch(qxk_sym_half(Kp.d(), Ks.d(), (int)nc, s), "sym");
-      check_launch(qxk_cart_to_ao_mat(&c->shells, Js.d(), J, s), "cart_to_ao");
-      check_launch(qxk_cart_to_ao_mat(&c->shells, Ks.d(), K, s), "cart_to_ao");
-      QX_HIP(hipStreamSynchronize(s));
+      check_launch(qxk_sym_half(Jp, Js, (int)nc, s), "sym");
+      check_launch(qxk_sym_half(Kp, Ks, (int)nc, s), "sym");
+      check_launch(qxk_cart_to_ao_mat(&c->shells, Js, J, s), "cart_to_ao");
+      check_launch(qxk_cart_to_ao_mat(&c->shells, Ks, K, s), "cart_to_ao");
     } else {
-      check_launch(qxk_sym_half(Jp.d(), J, (int)nc, s), "sym");
-      check_launch(qxk_sym_half(Kp.d(), K, (int)nc, s), "sym");
-      QX_HIP(hipStreamSynchronize(s));
+      check_launch(qxk_sym_half(Jp, J, (int)nc, s), "sym");
+      check_launch(qxk_sym_half(Kp, K, (int)nc, s), "sym");
     }
+    end_ws(c, s);
   } catch (const QxError& e) {
     return fail(c, e);
   }
@@ -899,12 +913,12 @@ int qx_eri_grad(qx_ctx* c, const double* D, double* grad, int precision, double 
   try {
     QX_HIP(hipSetDevice(c->device));
     hipStream_t s = (hipStream_t)stream;
-    const size_t nc = c->ncart;
-    std::unique_ptr<DevBuf> Dc(c->pure ? new DevBuf(nc * nc * sizeof(double)) : nullptr);
+    const size_t nc = c->ncart, n2 = nc * nc;
+    double* w = begin_ws(c, s);
     const double* Dcart = D;
     if (c->pure) {
-      check_launch(qxk_ao_to_cart_mat(&c->shells, D, Dc->d(), s), "ao_to_cart");
-      Dcart = Dc->d();
+      check_launch(qxk_ao_to_cart_mat(&c->shells, D, w + 2 * n2, s), "ao_to_cart");
+      Dcart = w + 2 * n2;
     }
     QxEpiDev e{};
     e.epi = QX_EPI_GRAD; e.nbf = (int)nc; e.natom = c->natom; e.coord = -1; e.D = Dcart; e.grad = grad;
@@ -917,7 +931,7 @@ int qx_eri_grad(qx_ctx* c, const double* D, double* grad, int precision, double 
       add_deriv_launches(c, c->work, e, 0, v);
     }
     run_launches(c, v, s, "eri_grad");
-    QX_HIP(hipStreamSynchronize(s));
+    end_ws(c, s);
   } catch (const QxError& e) {
     return fail(c, e);
   }

@@ -123,22 +123,19 @@ def restricted_hartree_fock(engine, nelectrons, options=None, return_aux_data=Fa
         m = len(diis_F)
         if m < 2:
             return F
-        B = torch.empty((m + 1, m + 1), dtype=dt, device=dev)
-        B[-1, :] = -1.0
-        B[:, -1] = -1.0
+        E = torch.stack([e.reshape(-1) for e in diis_e])
+        B = torch.full((m + 1, m + 1), -1.0, dtype=dt, device=dev)
+        # one fused reduction: a GEMM with m x m output and n^2-long inner
+        # dimension maps onto a single rocBLAS tile and takes milliseconds
+        B[:m, :m] = (E[:, None, :] * E[None, :, :]).sum(-1)
         B[-1, -1] = 0.0
-        for i in range(m):
-            for j in range(i, m):
-                v = torch.sum(diis_e[i] * diis_e[j])
-                B[i, j] = v
-                B[j, i] = v
         rhs = torch.zeros(m + 1, dtype=dt, device=dev)
         rhs[-1] = -1.0
         try:
             c = torch.linalg.solve(B, rhs)
         except Exception:
             return F
-        return sum(c[i] * diis_F[i] for i in range(m))
+        return torch.tensordot(c[:m], torch.stack(diis_F), dims=1)
 
     D = H.clone() if opts["guess_core"] else torch.zeros_like(H)
     J, K = build_jk(D)
