@@ -60,7 +60,12 @@ def kernel_table(eng):
             pf = (ctypes.c_double * 48)()
             pc = (ctypes.c_int * 48)()
             L.qx_class_flops(c, ctypes.byref(ef), pf, pc)
-            classes[c] = (tuple(l4), npart.value, ef.value, list(pf))
+            ngr = ctypes.c_int()
+            gf = (ctypes.c_double * 48)()
+            gp = (ctypes.c_int * 48)()
+            L.qx_class_groups(c, ctypes.byref(ngr), gf, gp)
+            classes[c] = (tuple(l4), ngr.value, ef.value, list(gf))
+        # derivative slots of the gradient pass are per group of parts
         l4, npart, ef, pf = classes[c]
         nm = "".join(map(str, l4))
         tp = "float" if f32.value else "double"
@@ -70,7 +75,7 @@ def kernel_table(eng):
             # J/K epilogue: 6 fused multiply-adds per integral
             info[slot] = (f"qx_kj_e{nm}<{tp}>", ef + 12.0 * N, nm, key)
         elif part.value < npart:
-            info[slot] = (f"qx_kg_d{nm}_{part.value}<{tp}>", pf[part.value], nm, key)
+            info[slot] = (f"qx_kg_d{nm}_g{part.value}<{tp}>", pf[part.value], nm, key)
         else:
             info[slot] = (f"unused slot {slot}", 0.0, nm, key)
     return info

@@ -163,6 +163,12 @@ int qx_class_quartets(const qx_ctx* ctx, long long* eri, long long* deri);
  * (adjoint-gradient epilogue included), and the number of derivative blocks
  * each part produces.  parts / ncomb hold up to 48 entries. */
 int qx_class_flops(int cls, double* eri, double* parts, int* ncomb);
+/* The adjoint-gradient pass launches one kernel per group of derivative
+ * parts of a class (blockIdx.y = part): number of groups, FP operations per
+ * primitive quartet of each group (sum over its parts) and parts per group;
+ * arrays of up to 48 entries.  Derivative profile slots of the gradient
+ * pass are indexed by group. */
+int qx_class_groups(int cls, int* ngroups, double* flops, int* nparts);
 int qx_profile_enable(qx_ctx* ctx, int on);
 int qx_profile_read(qx_ctx* ctx, double* ms, long long* launches, long long* pquartets);
 

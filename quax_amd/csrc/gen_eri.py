@@ -315,8 +315,8 @@ def emit_vrr_consts(L, vrr_lines):
         L.append("  const R rq = (R)(p * opq);")
 
 
-def _head(L, fn, sig, Lt, v, vn):
-    L.append(f"template <typename R>\n__device__ __forceinline__ void {fn}(const QxArgs& A_, int P1, int P2, "
+def _head(L, fn, sig, Lt, v, vn, ret="void", inline="__forceinline__"):
+    L.append(f"template <typename R>\n__device__ {inline} {ret} {fn}(const QxArgs& A_, int P1, int P2, "
              f"int u1, int u2{sig}) {{")
     emit_setup(L, Lt, False)
     emit_vrr_consts(L, v.lines)
@@ -466,10 +466,26 @@ def canonical_classes(lmax):
     return pairs, out
 
 
+# most derivative parts one adjoint-gradient launch (and source file) covers.
+# Grouping several parts under blockIdx.y would fill the GPU for classes with
+# few quartets, but hipcc's time on such a file grows far faster than the
+# number of parts (12 (dd|dd) parts: 24 minutes inlined, 12 minutes as
+# separate functions, against ~7 s per part alone), so every part keeps its
+# own kernel (the launch table and ABI still speak of groups of one).
+GROUP_PARTS = 1
+
+
+def part_groups(nparts):
+    """Contiguous, nearly equal groups of at most GROUP_PARTS parts."""
+    ng = -(-nparts // GROUP_PARTS)
+    return [list(range(i * nparts // ng, (i + 1) * nparts // ng)) for i in range(ng)]
+
+
 def emit_class_files(cls, outdir, fp32):
     """Writes, per class: cls_<l>_e.hip (array-form ERI kernel),
-    cls_<l>_j.hip (streaming J/K kernel) and, per derivative part,
-    cls_<l>_d<k>.hip (array form) and cls_<l>_g<k>.hip (streaming adjoint)."""
+    cls_<l>_j.hip (streaming J/K kernel), per derivative part
+    cls_<l>_d<k>.hip (array form) and, per group of parts, cls_<l>_g<k>.hip
+    (streaming adjoint kernel, blockIdx.y = part within the group)."""
     la, lb, lc, ld = cls
     nm = "".join(map(str, cls))
     dims = f"{ncart(la)}, {ncart(lb)}, {ncart(lc)}, {ncart(ld)}"
@@ -487,13 +503,25 @@ def emit_class_files(cls, outdir, fp32):
     write(f"cls_{nm}_j.hip", [src_j, "", f"QX_DEFINE_JK_KERNEL(e{nm}, {dims}, {1 if fp32 else 0})"])
     stats.append((cls, "eri", st))
     parts = deriv_parts(cls)
+    gsrc = []
     for pi, part in enumerate(parts):
         src_d, src_g, st, (cmask, xmask, dinv, NC, a0, a1, b0, b1) = emit_deriv_core(cls, pi, part)
+        assert (cmask, xmask, dinv, NC) == (7, 7, True, 9)
         args = f"{N}, {dims}, {cmask}, {xmask}, {1 if dinv else 0}, {NC}"
         write(f"cls_{nm}_d{pi}.hip", [src_d, "", f"QX_DEFINE_DERI_KERNEL(d{nm}_{pi}, {args}, {a0}, {a1}, {b0}, {b1})"])
-        write(f"cls_{nm}_g{pi}.hip", [src_g, "", f"QX_DEFINE_GRAD_KERNEL(d{nm}_{pi}, {args}, {1 if fp32 else 0})"])
+        gsrc.append(src_g)
         stats.append((cls, f"deriv{pi}", st))
-    return files, stats, len(parts)
+    groups = part_groups(len(parts))
+    for gi, members in enumerate(groups):
+        (pi,) = members
+        kern = ["template <typename R>",
+                f"__global__ __launch_bounds__(QX_BLOCK) void qx_kg_d{nm}_g{gi}(QxArgs A_) {{",
+                "  __shared__ double sg[QX_GRAD_LDS];",
+                "  qx_grad_body(A_, sg, [&](int P1, int P2, int u1, int u2, double (&g)[QX_GRAD_NCOMB]) {",
+                f"    qx_gr_d{nm}_{pi}<R>(A_, P1, P2, u1, u2, g);", "  });", "}",
+                f"QX_DEFINE_GRAD_LAUNCHER(d{nm}_g{gi}, {len(members)}, {1 if fp32 else 0})"]
+        write(f"cls_{nm}_g{gi}.hip", [gsrc[pi] for pi in members] + [""] + kern)
+    return files, stats, len(parts), groups
 
 
 def main():
@@ -502,12 +530,13 @@ def main():
     fp32 = "--no-fp32" not in sys.argv
     os.makedirs(outdir, exist_ok=True)
     _, classes = canonical_classes(lmax)
-    files, allstats, nparts, flops = [], [], {}, {}
+    files, allstats, nparts, flops, groups = [], [], {}, {}, {}
     for cls in classes:
-        f, s, np_ = emit_class_files(cls, outdir, fp32)
+        f, s, np_, grp = emit_class_files(cls, outdir, fp32)
         files += f
         allstats += s
         nparts[cls] = np_
+        groups[cls] = grp
         flops[cls] = (s[0][2]["flops"], [st["flops"] for _, _, st in s[1:]], [st["ncomb"] for _, _, st in s[1:]])
     D = ["// generated: kernel launch tables indexed by canonical class id", "#pragma once",
          f"#define QX_GEN_LMAX {lmax}", f"#define QX_NUM_CLASSES {len(classes)}",
@@ -519,18 +548,21 @@ def main():
         D.append(f'extern "C" int qx_launchj_e{nm}(const QxArgs*, const QxLaunch*);')
         for pi in range(nparts[cls]):
             D.append(f'extern "C" int qx_launch_d{nm}_{pi}(const QxArgs*, const QxLaunch*);')
-            D.append(f'extern "C" int qx_launchg_d{nm}_{pi}(const QxArgs*, const QxLaunch*);')
+        for gi in range(len(groups[cls])):
+            D.append(f'extern "C" int qx_launchg_d{nm}_g{gi}(const QxArgs*, const QxLaunch*);')
     D.append("")
     D.append("static const QxClassEntry qx_class_table[QX_NUM_CLASSES] = {")
     for cls in classes:
         nm = "".join(map(str, cls))
         dl = ", ".join(f"qx_launch_d{nm}_{pi}" for pi in range(nparts[cls]))
-        gl = ", ".join(f"qx_launchg_d{nm}_{pi}" for pi in range(nparts[cls]))
+        gl = ", ".join(f"qx_launchg_d{nm}_g{gi}" for gi in range(len(groups[cls])))
         ef, pf, pc = flops[cls]
         pfs = ", ".join(f"{x:.1f}" for x in pf)
         pcs = ", ".join(str(x) for x in pc)
+        gfs = ", ".join(f"{sum(pf[p] for p in g):.1f}" for g in groups[cls])
+        gps = ", ".join(str(len(g)) for g in groups[cls])
         D.append(f"  {{{cls[0]}, {cls[1]}, {cls[2]}, {cls[3]}, qx_launch_e{nm}, qx_launchj_e{nm}, {nparts[cls]}, "
-                 f"{{{dl}}}, {{{gl}}}, {ef:.1f}, {{{pfs}}}, {{{pcs}}}}},")
+                 f"{{{dl}}}, {len(groups[cls])}, {{{gl}}}, {ef:.1f}, {{{pfs}}}, {{{pcs}}}, {{{gfs}}}, {{{gps}}}}},")
     D.append("};")
     with open(os.path.join(outdir, "qx_classes.inc"), "w") as fh:
         fh.write("\n".join(D) + "\n")

@@ -77,6 +77,7 @@ struct QxEpiDev {
   double* out;
   const double* D;           // density, Cartesian nbf x nbf (JK, GRAD)
   double* J; double* K;      // unsymmetrised J', K' (JK)
+  int krows;                 // JK: stage K' rows of a block-uniform bra pair in LDS
   double* grad;              // 3N accumulators (GRAD)
 };
 
@@ -199,8 +200,11 @@ __device__ __forceinline__ void qx_scatter8(double* out, size_t n, size_t I, siz
 // ---- streaming-kernel epilogues -----------------------------------------
 // Flushes per-thread J'/K' partials: J'_ab += 4f sum_cd (ab|cd) D_cd etc.
 // Every lane calls this (converged); inactive lanes hold zeros.
+// sk != null: the whole block shares the bra pair, and K'_ac, K'_ad, K'_bc,
+// K'_bd go to its LDS rows (a: 0..NA-1, b: NA..NA+NB-1; columns are
+// functions) instead of global atomics.
 template <int NA, int NB, int NC, int ND>
-__device__ __forceinline__ void qx_jk_flush(const QxArgs& A_, int P1, int P2, bool active,
+__device__ __forceinline__ void qx_jk_flush(const QxArgs& A_, int P1, int P2, bool active, double* sk,
                                             const double (&jab)[NA * NB], const double (&jcd)[NC * ND],
                                             const double (&kac)[NA * NC], const double (&kad)[NA * ND],
                                             const double (&kbc)[NB * NC], const double (&kbd)[NB * ND]) {
@@ -228,6 +232,17 @@ __device__ __forceinline__ void qx_jk_flush(const QxArgs& A_, int P1, int P2, bo
   if (!active) return;
 #pragma unroll
   for (int k = 0; k < NC * ND; ++k) qx_atomic_add(&J[(ic + k / ND) * n + id + k % ND], fj * jcd[k]);
+  if (sk) {
+#pragma unroll
+    for (int k = 0; k < NA * NC; ++k) atomicAdd(&sk[(k / NC) * n + ic + k % NC], fk * kac[k]);
+#pragma unroll
+    for (int k = 0; k < NA * ND; ++k) atomicAdd(&sk[(k / ND) * n + id + k % ND], fk * kad[k]);
+#pragma unroll
+    for (int k = 0; k < NB * NC; ++k) atomicAdd(&sk[(NA + k / NC) * n + ic + k % NC], fk * kbc[k]);
+#pragma unroll
+    for (int k = 0; k < NB * ND; ++k) atomicAdd(&sk[(NA + k / ND) * n + id + k % ND], fk * kbd[k]);
+    return;
+  }
 #pragma unroll
   for (int k = 0; k < NA * NC; ++k) qx_atomic_add(&K[(ia + k / NC) * n + ic + k % NC], fk * kac[k]);
 #pragma unroll
@@ -386,31 +401,73 @@ __device__ __forceinline__ void qx_epi_deri(const QxArgs& A_, int P1, int P2,
     return (int)hipGetLastError();                                                                \
   }
 
+// K' rows staged in LDS when a block's quartets share one bra pair: for
+// classes with at least QX_JK_ROWS_MIN K' terms per quartet and rows of at
+// most QX_JK_LDS_MAX bytes.  The block's K' updates then cost LDS atomics
+// plus one coalesced pass of global atomics over the touched row entries.
+#define QX_JK_ROWS_MIN 4
+#define QX_JK_LDS_MAX 65536
+
 // Streaming J/K kernel: every integral is contracted with the density as
 // soon as it is formed; the per-thread J'/K' partials are flushed with
 // atomics, J'_ab summed across the wavefront when all its lanes share the
-// bra pair.  Cross-lane operations run in converged control flow.
+// bra pair, K' through LDS rows when the whole block does.  Cross-lane
+// operations and barriers run in converged, block-uniform control flow.
 #define QX_DEFINE_JK_KERNEL(NAME, NA, NB, NC, ND, WITH32)                                         \
   template <typename R>                                                                           \
   __global__ __launch_bounds__(QX_BLOCK) void qx_kj_##NAME(QxArgs A_) {                           \
+    HIP_DYNAMIC_SHARED(double, qx_sk)                                                             \
     const long long t = (long long)blockIdx.x * QX_BLOCK + threadIdx.x;                          \
     const bool active = t < A_.w.nquart;                                                          \
     int u1 = 0, u2 = 0, P1 = -1, P2 = -1;                                                         \
     if (active) qx_locate(A_, t, u1, u2, P1, P2);                                                 \
+    const int n_ = A_.e.nbf;                                                                      \
+    double* sk = nullptr;                                                                         \
+    int ra = 0, rb = 0;                                                                           \
+    if (A_.e.krows) { /* same result in every thread: block-uniform */                            \
+      const long long t0 = (long long)blockIdx.x * QX_BLOCK;                                      \
+      const long long t1 = min(t0 + QX_BLOCK, A_.w.nquart) - 1;                                   \
+      int v1, v2, B0, B1, w2;                                                                     \
+      qx_locate(A_, t0, v1, v2, B0, w2);                                                          \
+      qx_locate(A_, t1, v1, v2, B1, w2);                                                          \
+      if (B0 == B1) {                                                                             \
+        sk = qx_sk;                                                                               \
+        ra = A_.pr.bfa[B0];                                                                       \
+        rb = A_.pr.bfb[B0];                                                                       \
+        for (int i = threadIdx.x; i < (NA + NB) * n_; i += QX_BLOCK) sk[i] = 0.0;                 \
+        __syncthreads();                                                                          \
+      }                                                                                           \
+    }                                                                                             \
     double jab[NA * NB] = {}, jcd[NC * ND] = {}, kac[NA * NC] = {}, kad[NA * ND] = {},            \
            kbc[NB * NC] = {}, kbd[NB * ND] = {};                                                  \
     if (active) qx_jk_##NAME<R>(A_, P1, P2, u1, u2, jab, jcd, kac, kad, kbc, kbd);                \
-    qx_jk_flush<NA, NB, NC, ND>(A_, P1, P2, active, jab, jcd, kac, kad, kbc, kbd);                \
+    qx_jk_flush<NA, NB, NC, ND>(A_, P1, P2, active, sk, jab, jcd, kac, kad, kbc, kbd);            \
+    if (sk) {                                                                                     \
+      __syncthreads();                                                                            \
+      for (int i = threadIdx.x; i < (NA + NB) * n_; i += QX_BLOCK) {                              \
+        const double v = sk[i];                                                                   \
+        if (v != 0.0) {                                                                           \
+          const int r = i / n_, col = i - r * n_;                                                 \
+          const int row = r < NA ? ra + r : rb + (r - NA);                                        \
+          qx_atomic_add(&A_.e.K[(size_t)row * n_ + col], v);                                      \
+        }                                                                                         \
+      }                                                                                           \
+    }                                                                                             \
   }                                                                                               \
   extern "C" int qx_launchj_##NAME(const QxArgs* a, const QxLaunch* L) {                          \
     if (a->w.nquart <= 0) return 0;                                                               \
     if (a->e.epi != QX_EPI_JK) return -1;                                                         \
+    QxArgs b = *a;                                                                                \
+    const size_t rows_bytes = (size_t)(NA + NB) * (size_t)a->e.nbf * sizeof(double);              \
+    b.e.krows = a->e.krows && (NA * NC + NA * ND + NB * NC + NB * ND) >= QX_JK_ROWS_MIN &&        \
+                rows_bytes <= QX_JK_LDS_MAX;                                                      \
+    const size_t sh = b.e.krows ? rows_bytes : 0;                                                 \
     if (L->fp32) {                                                                                \
       if (!(WITH32)) return -1;                                                                   \
       if constexpr (WITH32)                                                                       \
-        hipLaunchKernelGGL(qx_kj_##NAME<float>, QX_GRID(a->w.nquart), dim3(QX_BLOCK), 0, L->stream, *a); \
+        hipLaunchKernelGGL(qx_kj_##NAME<float>, QX_GRID(a->w.nquart), dim3(QX_BLOCK), sh, L->stream, b); \
     } else {                                                                                      \
-      hipLaunchKernelGGL(qx_kj_##NAME<double>, QX_GRID(a->w.nquart), dim3(QX_BLOCK), 0, L->stream, *a); \
+      hipLaunchKernelGGL(qx_kj_##NAME<double>, QX_GRID(a->w.nquart), dim3(QX_BLOCK), sh, L->stream, b); \
     }                                                                                             \
     return (int)hipGetLastError();                                                                \
   }
@@ -436,53 +493,60 @@ __device__ __forceinline__ void qx_epi_deri(const QxArgs& A_, int P1, int P2,
     return (int)hipGetLastError();                                                                \
   }
 
-// Streaming adjoint-gradient kernel: each derivative integral is contracted
-// with the density pair product when formed; per-thread sums over the 4
-// centres x 3 directions go to LDS and are flushed once per block.
-#define QX_DEFINE_GRAD_KERNEL(NAME, N, NA, NB, NC, ND, CMASK, XMASK, DINV, NCOMB, WITH32)         \
-  template <typename R>                                                                           \
-  __global__ __launch_bounds__(QX_BLOCK) void qx_kg_##NAME(QxArgs A_) {                           \
-    __shared__ double sg[QX_GRAD_LDS];                                                            \
-    const int ng = 3 * A_.e.natom;                                                                \
-    const bool lds = ng <= QX_GRAD_LDS;                                                           \
-    if (lds) {                                                                                    \
-      for (int i = threadIdx.x; i < ng; i += QX_BLOCK) sg[i] = 0.0;                               \
-      __syncthreads();                                                                            \
-    }                                                                                             \
-    const long long t = (long long)blockIdx.x * QX_BLOCK + threadIdx.x;                          \
-    if (t < A_.w.nquart) {                                                                        \
-      int u1, u2, P1, P2;                                                                         \
-      qx_locate(A_, t, u1, u2, P1, P2);                                                           \
-      if (!qx_single_atom(A_, P1, P2)) {                                                          \
-        double g[NCOMB];                                                                          \
-        _Pragma("unroll") for (int c = 0; c < NCOMB; ++c) g[c] = 0.0;                             \
-        qx_gr_##NAME<R>(A_, P1, P2, u1, u2, g);                                                   \
-        double gv[12];                                                                            \
-        qx_grad_spread<CMASK, XMASK, DINV, NCOMB>(8.0 * qx_degeneracy(A_, P1, P2), g, gv);        \
-        const int at[4] = {A_.pr.atom_a[P1], A_.pr.atom_b[P1], A_.pr.atom_a[P2], A_.pr.atom_b[P2]}; \
-        _Pragma("unroll") for (int k = 0; k < 12; ++k) {                                          \
-          if (!(((CMASK | (DINV ? 8 : 0)) >> (k / 3)) & 1)) continue;                             \
-          const int slot = 3 * at[k / 3] + k % 3;                                                 \
-          if (lds) atomicAdd(&sg[slot], gv[k]);                                                   \
-          else qx_atomic_add(&A_.e.grad[slot], gv[k]);                                            \
-        }                                                                                         \
-      }                                                                                           \
-    }                                                                                             \
-    if (lds) {                                                                                    \
-      __syncthreads();                                                                            \
-      for (int i = threadIdx.x; i < ng; i += QX_BLOCK)                                            \
-        if (sg[i] != 0.0) qx_atomic_add(&A_.e.grad[i], sg[i]);                                    \
-    }                                                                                             \
-  }                                                                                               \
+// Streaming adjoint-gradient kernel body: each derivative integral is
+// contracted with the density pair product when formed; per-thread sums over
+// the 4 centres x 3 directions go to LDS and are flushed once per block.  A
+// generated kernel covers a group of derivative parts of one class (all with
+// d/dA, d/dB, d/dC and d/dD by invariance); its grid is (quartet blocks,
+// parts in the group) and `part` dispatches on blockIdx.y.  This build emits
+// groups of one part.
+#define QX_GRAD_NCOMB 9
+template <class Part>
+__device__ __forceinline__ void qx_grad_body(const QxArgs& A_, double* sg, Part part) {
+  const int ng = 3 * A_.e.natom;
+  const bool lds = ng <= QX_GRAD_LDS;
+  if (lds) {
+    for (int i = threadIdx.x; i < ng; i += QX_BLOCK) sg[i] = 0.0;
+    __syncthreads();
+  }
+  const long long t = (long long)blockIdx.x * QX_BLOCK + threadIdx.x;
+  if (t < A_.w.nquart) {
+    int u1, u2, P1, P2;
+    qx_locate(A_, t, u1, u2, P1, P2);
+    if (!qx_single_atom(A_, P1, P2)) {
+      double g[QX_GRAD_NCOMB];
+#pragma unroll
+      for (int c = 0; c < QX_GRAD_NCOMB; ++c) g[c] = 0.0;
+      part(P1, P2, u1, u2, g);
+      double gv[12];
+      qx_grad_spread<7, 7, 1, QX_GRAD_NCOMB>(8.0 * qx_degeneracy(A_, P1, P2), g, gv);
+      const int at[4] = {A_.pr.atom_a[P1], A_.pr.atom_b[P1], A_.pr.atom_a[P2], A_.pr.atom_b[P2]};
+#pragma unroll
+      for (int k = 0; k < 12; ++k) {
+        const int slot = 3 * at[k / 3] + k % 3;
+        if (lds) atomicAdd(&sg[slot], gv[k]);
+        else qx_atomic_add(&A_.e.grad[slot], gv[k]);
+      }
+    }
+  }
+  if (lds) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < ng; i += QX_BLOCK)
+      if (sg[i] != 0.0) qx_atomic_add(&A_.e.grad[i], sg[i]);
+  }
+}
+
+// Launcher of a generated group kernel qx_kg_NAME with NPARTS parts.
+#define QX_DEFINE_GRAD_LAUNCHER(NAME, NPARTS, WITH32)                                             \
   extern "C" int qx_launchg_##NAME(const QxArgs* a, const QxLaunch* L) {                          \
     if (a->w.nquart <= 0) return 0;                                                               \
     if (a->e.epi != QX_EPI_GRAD) return -1;                                                       \
+    const dim3 grid(QX_GRID(a->w.nquart).x, NPARTS);                                              \
     if (L->fp32) {                                                                                \
       if (!(WITH32)) return -1;                                                                   \
-      if constexpr (WITH32)                                                                       \
-        hipLaunchKernelGGL(qx_kg_##NAME<float>, QX_GRID(a->w.nquart), dim3(QX_BLOCK), 0, L->stream, *a); \
+      if constexpr (WITH32) hipLaunchKernelGGL(qx_kg_##NAME<float>, grid, dim3(QX_BLOCK), 0, L->stream, *a); \
     } else {                                                                                      \
-      hipLaunchKernelGGL(qx_kg_##NAME<double>, QX_GRID(a->w.nquart), dim3(QX_BLOCK), 0, L->stream, *a); \
+      hipLaunchKernelGGL(qx_kg_##NAME<double>, grid, dim3(QX_BLOCK), 0, L->stream, *a);           \
     }                                                                                             \
     return (int)hipGetLastError();                                                                \
   }
@@ -494,9 +558,12 @@ struct QxClassEntry {
   int (*eri)(const QxArgs*, const QxLaunch*);      // array form (store, scatter, Schwarz)
   int (*jk)(const QxArgs*, const QxLaunch*);       // streaming J/K
   int nparts;
-  int (*deri[QX_DPARTS_CAP])(const QxArgs*, const QxLaunch*);  // array form (derivative tensor)
-  int (*grad[QX_DPARTS_CAP])(const QxArgs*, const QxLaunch*);  // streaming adjoint gradient
-  double eri_flops;                  // algorithmic FP operations of one ERI body + J/K epilogue
-  double part_flops[QX_DPARTS_CAP];  // same per derivative part, with the adjoint epilogue
-  int part_ncomb[QX_DPARTS_CAP];     // derivative blocks (centre x direction) per part
+  int (*deri[QX_DPARTS_CAP])(const QxArgs*, const QxLaunch*);  // array form (derivative tensor), per part
+  int ngroups;
+  int (*grad[QX_DPARTS_CAP])(const QxArgs*, const QxLaunch*);  // streaming adjoint gradient, per part group
+  double eri_flops;                   // algorithmic FP operations of one ERI body (J/K epilogue excluded)
+  double part_flops[QX_DPARTS_CAP];   // same per derivative part, with the adjoint epilogue
+  int part_ncomb[QX_DPARTS_CAP];      // derivative blocks (centre x direction) per part
+  double group_flops[QX_DPARTS_CAP];  // sum of part_flops over each group
+  int group_parts[QX_DPARTS_CAP];     // parts per group
 };

@@ -770,13 +770,17 @@ static void add_deriv_launches(qx_ctx* c, const std::vector<QxWorkHost>& work, c
   for (const auto& wh : work) {
     if (wh.w.nquart <= 0) continue;
     const QxClassEntry& ent = qx_class_table[wh.cls];
-    for (int k = 0; k < ent.nparts; ++k) {
+    // adjoint gradient: one launch per group of parts; tensor: one per part
+    const bool grad = e.epi == QX_EPI_GRAD;
+    const int nl = grad ? ent.ngroups : ent.nparts;
+    for (int k = 0; k < nl; ++k) {
       QxArgs a;
       a.pr = c->pr; a.pp = c->pp; a.w = wh.w; a.e = e; a.boys = c->d_boys;
       const int slot = QX_SLOT_DERIV(wh.cls, k, fp32 ? 1 : 0);
-      auto fn = e.epi == QX_EPI_GRAD ? ent.grad[k] : ent.deri[k];
+      auto fn = grad ? ent.grad[k] : ent.deri[k];
       const long long nq = wh.w.nquart;
-      out.push_back({(double)nq * ent.part_flops[k], [c, a, fn, fp32, slot, nq](hipStream_t st) {
+      const double fl = grad ? ent.group_flops[k] : ent.part_flops[k];
+      out.push_back({(double)nq * fl, [c, a, fn, fp32, slot, nq](hipStream_t st) {
                        QxLaunch L{fp32, st};
                        return timed_launch(c, slot, nq, st, [&] { return fn(&a, &L); });
                      }});
@@ -881,6 +885,7 @@ int qx_jk_build(qx_ctx* c, const double* D, double* J, double* K, int precision,
     check_launch(qxk_fill(Kp, (long long)n2, 0.0, s), "fill");
     QxEpiDev e{};
     e.epi = QX_EPI_JK; e.nbf = (int)nc; e.natom = c->natom; e.coord = -1; e.D = Dcart; e.J = Jp; e.K = Kp;
+    e.krows = 1;  // each class kernel decides whether its K' rows fit in LDS
     std::vector<Launch> v;
     if (precision == QX_PREC_MIXED) {
       build_split(c, thr);
@@ -1081,6 +1086,17 @@ int qx_class_flops(int cls, double* eri, double* parts, int* ncomb) {
   for (int k = 0; k < e.nparts; ++k) {
     if (parts) parts[k] = e.part_flops[k];
     if (ncomb) ncomb[k] = e.part_ncomb[k];
+  }
+  return QX_OK;
+}
+
+int qx_class_groups(int cls, int* ngroups, double* flops, int* nparts) {
+  if (cls < 0 || cls >= QX_NUM_CLASSES) return QX_EINVAL;
+  const QxClassEntry& e = qx_class_table[cls];
+  if (ngroups) *ngroups = e.ngroups;
+  for (int k = 0; k < e.ngroups; ++k) {
+    if (flops) flops[k] = e.group_flops[k];
+    if (nparts) nparts[k] = e.group_parts[k];
   }
   return QX_OK;
 }

@@ -27,7 +27,8 @@ void lane_main(int lane) {
   while (true) {
     P.bar->arrive_and_wait();  // start of a block (or shutdown)
     if (P.quit.load()) return;
-    blockIdx.x = P.cur_block;
+    blockIdx.x = P.cur_block % P.grid.x;
+    blockIdx.y = P.cur_block / P.grid.x;
     blockDim = P.block;
     gridDim = P.grid;
     if ((unsigned)lane < P.block.x) P.body();
@@ -73,7 +74,7 @@ void run_grid(dim3 grid, dim3 block, const std::function<void()>& body) {
   P.body = body;
   P.grid = grid;
   P.block = block;
-  for (unsigned b = 0; b < grid.x; ++b) {
+  for (unsigned b = 0; b < grid.x * grid.y; ++b) {
     P.cur_block = b;
     P.bar->arrive_and_wait();
     P.bar->arrive_and_wait();

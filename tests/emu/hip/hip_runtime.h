@@ -118,11 +118,29 @@ inline double unsafeAtomicAdd(double* p, double v) {
 inline double atomicAdd(double* p, double v) { return std::atomic_ref<double>(*p).fetch_add(v); }
 inline float atomicAdd(float* p, float v) { return std::atomic_ref<float>(*p).fetch_add(v); }
 
-template <class K, class... A>
-inline void emu_launch(K kernel, dim3 grid, dim3 block, A... args) {
-  emu::run_grid(grid, block, [=]() { kernel(args...); });
+namespace emu {
+// dynamic shared memory of the running grid: exactly the launch's byte
+// count, so AddressSanitizer flags any access past it
+inline std::vector<double*>& dyn_pool() {
+  static std::vector<double*> p(1, nullptr);
+  return p;
 }
-#define hipLaunchKernelGGL(kernel, grid, block, shmem, stream, ...) emu_launch(kernel, grid, block, __VA_ARGS__)
+inline void* dyn_shared() { return dyn_pool()[0]; }
+}  // namespace emu
+#define HIP_DYNAMIC_SHARED(type, var) type* var = reinterpret_cast<type*>(emu::dyn_shared());
+
+template <class K, class... A>
+inline void emu_launch(K kernel, dim3 grid, dim3 block, size_t shmem, A... args) {
+  double*& d = emu::dyn_pool()[0];
+  delete[] d;
+  d = shmem ? new double[(shmem + 7) / 8] : nullptr;
+  if (shmem && std::getenv("EMU_TRACE_SHMEM")) std::fprintf(stderr, "emu: launch with %zu B dynamic LDS\n", shmem);
+  emu::run_grid(grid, block, [=]() { kernel(args...); });
+  delete[] d;
+  d = nullptr;
+}
+#define hipLaunchKernelGGL(kernel, grid, block, shmem, stream, ...) \
+  emu_launch(kernel, grid, block, (size_t)(shmem), __VA_ARGS__)
 
 using std::max;
 using std::min;

@@ -93,6 +93,12 @@ def test_class_table_static_info():
         pc = (ctypes.c_int * 48)()
         assert L.qx_class_flops(k, ctypes.byref(ef), pf, pc) == 0
         assert ef.value > 0 and all(pf[i] > 0 for i in range(npart.value))
+        ng = ctypes.c_int()
+        gf = (ctypes.c_double * 48)()
+        gp = (ctypes.c_int * 48)()
+        assert L.qx_class_groups(k, ctypes.byref(ng), gf, gp) == 0
+        assert 1 <= ng.value <= npart.value and sum(gp[i] for i in range(ng.value)) == npart.value
+        assert abs(sum(gf[i] for i in range(ng.value)) - sum(pf[i] for i in range(npart.value))) < 1e-6 * gf[0] * 48
     assert len(seen) == 21
 
 
