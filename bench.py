@@ -64,21 +64,38 @@ def kernel_table(eng):
             gf = (ctypes.c_double * 48)()
             gp = (ctypes.c_int * 48)()
             L.qx_class_groups(c, ctypes.byref(ngr), gf, gp)
-            classes[c] = (tuple(l4), ngr.value, ef.value, list(gf))
+            njk = ctypes.c_int()
+            jf = (ctypes.c_double * 8)()
+            L.qx_class_jk(c, ctypes.byref(njk), jf)
+            classes[c] = (tuple(l4), ngr.value, ef.value, list(gf), njk.value, list(jf))
         # derivative slots of the gradient pass are per group of parts
-        l4, npart, ef, pf = classes[c]
+        l4, npart, ef, pf, njk, jf = classes[c]
         nm = "".join(map(str, l4))
         tp = "float" if f32.value else "double"
         N = ncart(l4[0]) * ncart(l4[1]) * ncart(l4[2]) * ncart(l4[3])
-        key = ("jk" if kind.value == 0 else "deriv") + ("32" if f32.value else "64")
+        key = ("eri", "deriv", "jk")[kind.value] + ("32" if f32.value else "64")
         if kind.value == 0:
-            # J/K epilogue: 6 fused multiply-adds per integral
-            info[slot] = (f"qx_kj_e{nm}<{tp}>", ef + 12.0 * N, nm, key)
+            info[slot] = (f"qx_k_e{nm}<{tp}>", ef, nm, key)
+        elif kind.value == 2:
+            info[slot] = (f"qx_kj_e{nm}_{part.value}<{tp}>", jf[part.value] if part.value < njk else 0.0, nm, key)
         elif part.value < npart:
             info[slot] = (f"qx_kg_d{nm}_g{part.value}<{tp}>", pf[part.value], nm, key)
         else:
             info[slot] = (f"unused slot {slot}", 0.0, nm, key)
     return info
+
+
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch that counter runs recorded for this kernel on this
+    workload (profiles/hbm_traffic.txt), else None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 3 and not line.startswith("#") and f[0] == workload and f[1] == kernel:
+            return float(f[2])
+    return None
 
 
 def class_times(table, ms, steps):
@@ -272,11 +289,15 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    # first-launch costs (code-object loading, side streams) stay out of the timings
+    # first-launch costs (code-object loading, side streams, the solver
+    # libraries behind torch.linalg) stay out of the timings
     S0, T0, V0 = eng.oei()
     for prec in ((0,) if args.no_mixed else (0, 1)):
         eng.jk(S0, precision=prec, fp32_threshold=thr)
         eng.eri_grad(S0, precision=prec, fp32_threshold=thr)
+    if not args.no_scf:
+        from quax_amd.methods.hartree_fock import restricted_hartree_fock
+        restricted_hartree_fock(eng, mol.nelectron(), {"diis": True, "spectral_shift": False, "maxit": 3})
     torch.cuda.synchronize()
 
     hf = {"fp32_threshold": thr, "fp32_quartet_fraction": n32 / max(n64 + n32, 1)}
@@ -374,7 +395,9 @@ def main():
                    "one_stream_ms_per_step": dts * 1e3 / args.steps,
                    "eri_kernel_ms": eri_ms, "deri_kernel_ms": deri_ms},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64, "traffic": None, "kernel": name, "avg_launch_ms": avg_ms,
+                     "frac": achieved / PEAK_FP64, "traffic": measured_traffic(args.workload, name),
+                     "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated)",
+                     "kernel": name, "avg_launch_ms": avg_ms,
                      "flops_per_launch": flops_per_launch, "step_flops": total_flops,
                      "step_tflops": total_flops / (ms_step * 1e-3) / 1e12},
         "cpu_baseline": cpu,

@@ -151,8 +151,9 @@ int qx_set_streams(qx_ctx* ctx, int nstreams);
  * synchronises and returns, per
  * slot, the summed kernel time in ms, the launch count and the primitive-pair
  * quartets processed.  Arrays hold qx_profile_slots() entries;
- * qx_profile_slot names a slot: kind 0 = ERI / J-K kernel of class cls,
- * kind 1 = derivative part `part` of class cls; fp32 = FP32 variant. */
+ * qx_profile_slot names a slot: kind 0 = array-form ERI kernel of class
+ * cls, kind 1 = derivative part group `part` of class cls, kind 2 = J/K part
+ * `part` of class cls; fp32 = FP32 variant. */
 int qx_num_classes(void);
 int qx_profile_slots(void);
 int qx_profile_slot(int slot, int* kind, int* cls, int* part, int* fp32);
@@ -169,6 +170,10 @@ int qx_class_flops(int cls, double* eri, double* parts, int* ncomb);
  * arrays of up to 48 entries.  Derivative profile slots of the gradient
  * pass are indexed by group. */
 int qx_class_groups(int cls, int* ngroups, double* flops, int* nparts);
+/* The J/K pass launches one kernel per range of a components of a class:
+ * number of parts and FP operations per primitive quartet of each (J/K
+ * epilogue included); flops holds up to 8 entries. */
+int qx_class_jk(int cls, int* nparts, double* flops);
 int qx_profile_enable(qx_ctx* ctx, int on);
 int qx_profile_read(qx_ctx* ctx, double* ms, long long* launches, long long* pquartets);
 

@@ -64,6 +64,7 @@ SIGNATURES = {
     "qx_class_quartets": (_i, [_vp, ctypes.POINTER(_ll), ctypes.POINTER(_ll)]),
     "qx_class_flops": (_i, [_i, _dp, _dp, _ip]),
     "qx_class_groups": (_i, [_i, _ip, _dp, _ip]),
+    "qx_class_jk": (_i, [_i, _ip, _dp]),
     "qx_profile_enable": (_i, [_vp, _i]),
     "qx_profile_read": (_i, [_vp, _dp, ctypes.POINTER(_ll), ctypes.POINTER(_ll)]),
 }

@@ -234,6 +234,30 @@ def deriv_targets(l, part):
     return T
 
 
+# largest SSA node count of one streaming J/K body; above it the class is
+# split over ranges of a components (as the derivative parts are).  Measured
+# on cluster10: splitting (dd|dp) into 3 and (dd|dd) into 6 parts made their
+# J/K kernels 1.2x and 1.8x slower (recomputed recurrences cost more than
+# the 5-11 KB of spills saved), so no current class is split.
+JK_NODES = 8000
+
+
+def jk_parts(l, budget=None):
+    budget = budget or JK_NODES
+    la, lb, lc, ld = l
+    NA = ncart(la)
+
+    def nodes(a0, a1):
+        T = [(a, b, c, d) for a in comps(la)[a0:a1] for b in comps(lb) for c in comps(lc) for d in comps(ld)]
+        h, v, _, _, _ = build_dag(T)
+        return len(h.lines) + len(v.lines)
+
+    for k in range(1, NA + 1):
+        rs = _ranges(NA, k)
+        if k == NA or all(nodes(*r) <= budget for r in rs):
+            return rs
+
+
 def _ranges(n, k):
     """k contiguous, nearly equal index ranges covering 0..n-1."""
     return [(i * n // k, (i + 1) * n // k) for i in range(k)]
@@ -343,29 +367,37 @@ def emit_eri_core(l):
     groups = [(pos[k], [f"  o[{k}] = {outs[k]};"]) for k in range(N)]
     L += ["  " + x if not x.startswith("  ") else x for x in interleave(h.lines, groups)]
     L.append("}")
-    # streaming J/K form
-    J = []
-    _head(J, f"qx_jk_e{nm}", ", double (&jab)[%d], double (&jcd)[%d], double (&kac)[%d], double (&kad)[%d], "
-          "double (&kbc)[%d], double (&kbd)[%d]" % (NA * NB, NC * ND, NA * NC, NA * ND, NB * NC, NB * ND), Lt, v, vn)
-    emit_dblocks(J, nn)
-    groups = []
-    k = 0
-    for i in range(NA):
-        for j in range(NB):
-            for kk in range(NC):
-                for ll in range(ND):
-                    ij, kl, ik, il, jk, jl = i * NB + j, kk * ND + ll, i * NC + kk, i * ND + ll, j * NC + kk, j * ND + ll
-                    groups.append((pos[k], [
-                        f"  {{ const double v_ = (double){outs[k]};",
-                        f"    jab[{ij}] += v_ * Dcd[{kl}]; jcd[{kl}] += v_ * Dab[{ij}];",
-                        f"    kac[{ik}] += v_ * Dbd[{jl}]; kad[{il}] += v_ * Dbc[{jk}];",
-                        f"    kbc[{jk}] += v_ * Dad[{il}]; kbd[{jl}] += v_ * Dac[{ik}]; }}"]))
-                    k += 1
-    J += ["  " + x if not x.startswith("  ") else x for x in interleave(h.lines, groups)]
-    J.append("}")
     stats = dict(vrr=len(v.lines), vrr_flops=v.nflop, hrr=len(h.lines), hrr_flops=h.nflop, L=Lt, N=N,
                  flops=core_flops(Lt, v.nflop, h.nflop))
-    return "\n".join(L), "\n".join(J), stats
+    # streaming J/K form, one body per range of a components
+    jks = []
+    for pi, (a0, a1) in enumerate(jk_parts(l)):
+        Tp = [t for t in T if comps(la).index(t[0]) in range(a0, a1)]
+        hp, vp, vnp, outp, posp = build_dag(Tp)
+        J = []
+        _head(J, f"qx_jk_e{nm}_{pi}", ", double (&jab)[%d], double (&jcd)[%d], double (&kac)[%d], double (&kad)[%d], "
+              "double (&kbc)[%d], double (&kbd)[%d]" % (NA * NB, NC * ND, NA * NC, NA * ND, NB * NC, NB * ND),
+              Lt, vp, vnp)
+        emit_dblocks(J, nn)
+        groups = []
+        k = 0
+        for i in range(a0, a1):
+            for j in range(NB):
+                for kk in range(NC):
+                    for ll in range(ND):
+                        ij, kl, ik, il, jk, jl = (i * NB + j, kk * ND + ll, i * NC + kk, i * ND + ll, j * NC + kk,
+                                                  j * ND + ll)
+                        groups.append((posp[k], [
+                            f"  {{ const double v_ = (double){outp[k]};",
+                            f"    jab[{ij}] += v_ * Dcd[{kl}]; jcd[{kl}] += v_ * Dab[{ij}];",
+                            f"    kac[{ik}] += v_ * Dbd[{jl}]; kad[{il}] += v_ * Dbc[{jk}];",
+                            f"    kbc[{jk}] += v_ * Dad[{il}]; kbd[{jl}] += v_ * Dac[{ik}]; }}"]))
+                        k += 1
+        J += ["  " + x if not x.startswith("  ") else x for x in interleave(hp.lines, groups)]
+        J.append("}")
+        # J/K epilogue: 6 fused multiply-adds per integral
+        jks.append(("\n".join(J), (a0, a1), core_flops(Lt, vp.nflop, hp.nflop) + 12 * k))
+    return "\n".join(L), jks, stats
 
 
 def emit_deriv_core(l, pi, part):
@@ -483,7 +515,7 @@ def part_groups(nparts):
 
 def emit_class_files(cls, outdir, fp32):
     """Writes, per class: cls_<l>_e.hip (array-form ERI kernel),
-    cls_<l>_j.hip (streaming J/K kernel), per derivative part
+    cls_<l>_j<k>.hip (streaming J/K kernel per range of a), per derivative part
     cls_<l>_d<k>.hip (array form) and, per group of parts, cls_<l>_g<k>.hip
     (streaming adjoint kernel, blockIdx.y = part within the group)."""
     la, lb, lc, ld = cls
@@ -498,9 +530,12 @@ def emit_class_files(cls, outdir, fp32):
             fh.write("\n".join(['#include "../qx_device.h"', ""] + body + [""]))
         files.append(path)
 
-    src_e, src_j, st = emit_eri_core(cls)
+    src_e, jks, st = emit_eri_core(cls)
     write(f"cls_{nm}_e.hip", [src_e, "", f"QX_DEFINE_ERI_KERNEL(e{nm}, {N}, {dims})"])
-    write(f"cls_{nm}_j.hip", [src_j, "", f"QX_DEFINE_JK_KERNEL(e{nm}, {dims}, {1 if fp32 else 0})"])
+    for pi, (src_j, (a0, a1), _) in enumerate(jks):
+        write(f"cls_{nm}_j{pi}.hip", [src_j, "", f"QX_DEFINE_JK_KERNEL(e{nm}_{pi}, {dims}, {1 if fp32 else 0}, "
+                                                  f"{a0}, {a1})"])
+    st["jk_flops"] = [f for _, _, f in jks]
     stats.append((cls, "eri", st))
     parts = deriv_parts(cls)
     gsrc = []
@@ -530,22 +565,26 @@ def main():
     fp32 = "--no-fp32" not in sys.argv
     os.makedirs(outdir, exist_ok=True)
     _, classes = canonical_classes(lmax)
-    files, allstats, nparts, flops, groups = [], [], {}, {}, {}
+    files, allstats, nparts, flops, groups, jkf = [], [], {}, {}, {}, {}
     for cls in classes:
         f, s, np_, grp = emit_class_files(cls, outdir, fp32)
         files += f
         allstats += s
         nparts[cls] = np_
         groups[cls] = grp
+        jkf[cls] = s[0][2]["jk_flops"]
         flops[cls] = (s[0][2]["flops"], [st["flops"] for _, _, st in s[1:]], [st["ncomb"] for _, _, st in s[1:]])
     D = ["// generated: kernel launch tables indexed by canonical class id", "#pragma once",
          f"#define QX_GEN_LMAX {lmax}", f"#define QX_NUM_CLASSES {len(classes)}",
          f"#define QX_MAX_DPARTS {max(nparts.values())}",
+         f"#define QX_MAX_JKPARTS {max(len(v) for v in jkf.values())}",
+         "static_assert(QX_MAX_JKPARTS <= QX_JKPARTS_CAP, \"J/K part table too small\");",
          "static_assert(QX_MAX_DPARTS <= QX_DPARTS_CAP, \"derivative part table too small\");", ""]
     for cls in classes:
         nm = "".join(map(str, cls))
         D.append(f'extern "C" int qx_launch_e{nm}(const QxArgs*, const QxLaunch*);')
-        D.append(f'extern "C" int qx_launchj_e{nm}(const QxArgs*, const QxLaunch*);')
+        for pi in range(len(jkf[cls])):
+            D.append(f'extern "C" int qx_launchj_e{nm}_{pi}(const QxArgs*, const QxLaunch*);')
         for pi in range(nparts[cls]):
             D.append(f'extern "C" int qx_launch_d{nm}_{pi}(const QxArgs*, const QxLaunch*);')
         for gi in range(len(groups[cls])):
@@ -561,8 +600,11 @@ def main():
         pcs = ", ".join(str(x) for x in pc)
         gfs = ", ".join(f"{sum(pf[p] for p in g):.1f}" for g in groups[cls])
         gps = ", ".join(str(len(g)) for g in groups[cls])
-        D.append(f"  {{{cls[0]}, {cls[1]}, {cls[2]}, {cls[3]}, qx_launch_e{nm}, qx_launchj_e{nm}, {nparts[cls]}, "
-                 f"{{{dl}}}, {len(groups[cls])}, {{{gl}}}, {ef:.1f}, {{{pfs}}}, {{{pcs}}}, {{{gfs}}}, {{{gps}}}}},")
+        jl = ", ".join(f"qx_launchj_e{nm}_{pi}" for pi in range(len(jkf[cls])))
+        jfs = ", ".join(f"{x:.1f}" for x in jkf[cls])
+        D.append(f"  {{{cls[0]}, {cls[1]}, {cls[2]}, {cls[3]}, qx_launch_e{nm}, {len(jkf[cls])}, {{{jl}}}, {{{jfs}}}, "
+                 f"{nparts[cls]}, {{{dl}}}, {len(groups[cls])}, {{{gl}}}, {ef:.1f}, {{{pfs}}}, {{{pcs}}}, {{{gfs}}}, "
+                 f"{{{gps}}}}},")
     D.append("};")
     with open(os.path.join(outdir, "qx_classes.inc"), "w") as fh:
         fh.write("\n".join(D) + "\n")

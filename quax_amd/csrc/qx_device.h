@@ -203,7 +203,9 @@ __device__ __forceinline__ void qx_scatter8(double* out, size_t n, size_t I, siz
 // sk != null: the whole block shares the bra pair, and K'_ac, K'_ad, K'_bc,
 // K'_bd go to its LDS rows (a: 0..NA-1, b: NA..NA+NB-1; columns are
 // functions) instead of global atomics.
-template <int NA, int NB, int NC, int ND>
+// Only rows a in [A0, A1) of J'_ab, K'_ac and K'_ad can be non-zero (a J/K
+// part covers that range of a components).
+template <int NA, int NB, int NC, int ND, int A0, int A1>
 __device__ __forceinline__ void qx_jk_flush(const QxArgs& A_, int P1, int P2, bool active, double* sk,
                                             const double (&jab)[NA * NB], const double (&jcd)[NC * ND],
                                             const double (&kac)[NA * NC], const double (&kad)[NA * ND],
@@ -221,22 +223,22 @@ __device__ __forceinline__ void qx_jk_flush(const QxArgs& A_, int P1, int P2, bo
   }
   if (uni) {
 #pragma unroll
-    for (int k = 0; k < NA * NB; ++k) {
+    for (int k = A0 * NB; k < A1 * NB; ++k) {
       const double v = qx_wave_sum(fj * jab[k]);
       if (lead) qx_atomic_add(&J[(ia + k / NB) * n + ib + k % NB], v);
     }
   } else if (active) {
 #pragma unroll
-    for (int k = 0; k < NA * NB; ++k) qx_atomic_add(&J[(ia + k / NB) * n + ib + k % NB], fj * jab[k]);
+    for (int k = A0 * NB; k < A1 * NB; ++k) qx_atomic_add(&J[(ia + k / NB) * n + ib + k % NB], fj * jab[k]);
   }
   if (!active) return;
 #pragma unroll
   for (int k = 0; k < NC * ND; ++k) qx_atomic_add(&J[(ic + k / ND) * n + id + k % ND], fj * jcd[k]);
   if (sk) {
 #pragma unroll
-    for (int k = 0; k < NA * NC; ++k) atomicAdd(&sk[(k / NC) * n + ic + k % NC], fk * kac[k]);
+    for (int k = A0 * NC; k < A1 * NC; ++k) atomicAdd(&sk[(k / NC) * n + ic + k % NC], fk * kac[k]);
 #pragma unroll
-    for (int k = 0; k < NA * ND; ++k) atomicAdd(&sk[(k / ND) * n + id + k % ND], fk * kad[k]);
+    for (int k = A0 * ND; k < A1 * ND; ++k) atomicAdd(&sk[(k / ND) * n + id + k % ND], fk * kad[k]);
 #pragma unroll
     for (int k = 0; k < NB * NC; ++k) atomicAdd(&sk[(NA + k / NC) * n + ic + k % NC], fk * kbc[k]);
 #pragma unroll
@@ -244,9 +246,9 @@ __device__ __forceinline__ void qx_jk_flush(const QxArgs& A_, int P1, int P2, bo
     return;
   }
 #pragma unroll
-  for (int k = 0; k < NA * NC; ++k) qx_atomic_add(&K[(ia + k / NC) * n + ic + k % NC], fk * kac[k]);
+  for (int k = A0 * NC; k < A1 * NC; ++k) qx_atomic_add(&K[(ia + k / NC) * n + ic + k % NC], fk * kac[k]);
 #pragma unroll
-  for (int k = 0; k < NA * ND; ++k) qx_atomic_add(&K[(ia + k / ND) * n + id + k % ND], fk * kad[k]);
+  for (int k = A0 * ND; k < A1 * ND; ++k) qx_atomic_add(&K[(ia + k / ND) * n + id + k % ND], fk * kad[k]);
 #pragma unroll
   for (int k = 0; k < NB * NC; ++k) qx_atomic_add(&K[(ib + k / NC) * n + ic + k % NC], fk * kbc[k]);
 #pragma unroll
@@ -413,7 +415,7 @@ __device__ __forceinline__ void qx_epi_deri(const QxArgs& A_, int P1, int P2,
 // atomics, J'_ab summed across the wavefront when all its lanes share the
 // bra pair, K' through LDS rows when the whole block does.  Cross-lane
 // operations and barriers run in converged, block-uniform control flow.
-#define QX_DEFINE_JK_KERNEL(NAME, NA, NB, NC, ND, WITH32)                                         \
+#define QX_DEFINE_JK_KERNEL(NAME, NA, NB, NC, ND, WITH32, A0, A1)                                 \
   template <typename R>                                                                           \
   __global__ __launch_bounds__(QX_BLOCK) void qx_kj_##NAME(QxArgs A_) {                           \
     HIP_DYNAMIC_SHARED(double, qx_sk)                                                             \
@@ -441,7 +443,7 @@ __device__ __forceinline__ void qx_epi_deri(const QxArgs& A_, int P1, int P2,
     double jab[NA * NB] = {}, jcd[NC * ND] = {}, kac[NA * NC] = {}, kad[NA * ND] = {},            \
            kbc[NB * NC] = {}, kbd[NB * ND] = {};                                                  \
     if (active) qx_jk_##NAME<R>(A_, P1, P2, u1, u2, jab, jcd, kac, kad, kbc, kbd);                \
-    qx_jk_flush<NA, NB, NC, ND>(A_, P1, P2, active, sk, jab, jcd, kac, kad, kbc, kbd);            \
+    qx_jk_flush<NA, NB, NC, ND, A0, A1>(A_, P1, P2, active, sk, jab, jcd, kac, kad, kbc, kbd);    \
     if (sk) {                                                                                     \
       __syncthreads();                                                                            \
       for (int i = threadIdx.x; i < (NA + NB) * n_; i += QX_BLOCK) {                              \
@@ -459,7 +461,7 @@ __device__ __forceinline__ void qx_epi_deri(const QxArgs& A_, int P1, int P2,
     if (a->e.epi != QX_EPI_JK) return -1;                                                         \
     QxArgs b = *a;                                                                                \
     const size_t rows_bytes = (size_t)(NA + NB) * (size_t)a->e.nbf * sizeof(double);              \
-    b.e.krows = a->e.krows && (NA * NC + NA * ND + NB * NC + NB * ND) >= QX_JK_ROWS_MIN &&        \
+    b.e.krows = a->e.krows && ((A1 - A0) * (NC + ND) + NB * NC + NB * ND) >= QX_JK_ROWS_MIN &&    \
                 rows_bytes <= QX_JK_LDS_MAX;                                                      \
     const size_t sh = b.e.krows ? rows_bytes : 0;                                                 \
     if (L->fp32) {                                                                                \
@@ -552,11 +554,14 @@ __device__ __forceinline__ void qx_grad_body(const QxArgs& A_, double* sg, Part 
   }
 
 #define QX_DPARTS_CAP 48  // most derivative parts one class may be split into
+#define QX_JKPARTS_CAP 8  // most J/K parts (ranges of a components) of one class
 
 struct QxClassEntry {
   int la, lb, lc, ld;
   int (*eri)(const QxArgs*, const QxLaunch*);      // array form (store, scatter, Schwarz)
-  int (*jk)(const QxArgs*, const QxLaunch*);       // streaming J/K
+  int njk;
+  int (*jk[QX_JKPARTS_CAP])(const QxArgs*, const QxLaunch*);  // streaming J/K, per part
+  double jk_flops[QX_JKPARTS_CAP];   // FP operations per primitive quartet, J/K epilogue included
   int nparts;
   int (*deri[QX_DPARTS_CAP])(const QxArgs*, const QxLaunch*);  // array form (derivative tensor), per part
   int ngroups;

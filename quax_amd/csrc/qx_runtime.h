@@ -103,12 +103,15 @@ struct qx_ctx {
   } split;
 };
 
-// profile slots: ERI / J-K kernel of class c -> QX_SLOT_ERI(c, fp32);
-// derivative part k of class c -> QX_SLOT_DERIV(c, k, fp32)
+// profile slots: array-form ERI kernel of class c -> QX_SLOT_ERI(c, fp32);
+// derivative part / group k of class c -> QX_SLOT_DERIV(c, k, fp32); J/K
+// part k of class c -> QX_SLOT_JK(c, k, fp32)
 #define QX_NUM_CLASSES_MAX 21
 #define QX_SLOT_ERI(c, f32) ((f32) * QX_NUM_CLASSES_MAX + (c))
 #define QX_SLOT_DERIV(c, k, f32) (2 * QX_NUM_CLASSES_MAX + ((f32) * QX_NUM_CLASSES_MAX + (c)) * QX_DPARTS_CAP + (k))
-#define QX_PROFILE_SLOTS (2 * QX_NUM_CLASSES_MAX * (1 + QX_DPARTS_CAP))
+#define QX_SLOT_JK0 (2 * QX_NUM_CLASSES_MAX * (1 + QX_DPARTS_CAP))
+#define QX_SLOT_JK(c, k, f32) (QX_SLOT_JK0 + ((f32) * QX_NUM_CLASSES_MAX + (c)) * QX_JKPARTS_CAP + (k))
+#define QX_PROFILE_SLOTS (QX_SLOT_JK0 + 2 * QX_NUM_CLASSES_MAX * QX_JKPARTS_CAP)
 
 // helper kernels (qx_kernels.hip)
 extern "C" {

@@ -211,8 +211,7 @@ int launch_eri(qx_ctx* c, const QxWorkDev& w, int cls, const QxEpiDev& e, int fp
   a.pr = pr; a.pp = pp; a.w = w; a.e = e; a.boys = c->d_boys;
   QxLaunch L{fp32, s};
   const QxClassEntry& ent = qx_class_table[cls];
-  auto fn = e.epi == QX_EPI_JK ? ent.jk : ent.eri;
-  return timed_launch(c, QX_SLOT_ERI(cls, fp32 ? 1 : 0), w.nquart, s, [&] { return fn(&a, &L); });
+  return timed_launch(c, QX_SLOT_ERI(cls, fp32 ? 1 : 0), w.nquart, s, [&] { return ent.eri(&a, &L); });
 }
 
 // One kernel launch of a pass, with its estimated cost (flops).
@@ -788,16 +787,23 @@ static void add_deriv_launches(qx_ctx* c, const std::vector<QxWorkHost>& work, c
   }
 }
 
-// J/K kernels of every class over one set of lists.
+// J/K kernels (every part of every class) over one set of lists.
 static void add_jk_launches(qx_ctx* c, const std::vector<QxWorkHost>& work, const QxEpiDev& e, int fp32,
                             std::vector<Launch>& out) {
   for (const auto& wh : work) {
     if (wh.w.nquart <= 0) continue;
-    const QxWorkDev w = wh.w;
-    const int cls = wh.cls;
-    out.push_back({(double)w.nquart * qx_class_table[cls].eri_flops, [c, w, cls, e, fp32](hipStream_t st) {
-                     return launch_eri(c, w, cls, e, fp32, st, c->pr, c->pp);
-                   }});
+    const QxClassEntry& ent = qx_class_table[wh.cls];
+    for (int k = 0; k < ent.njk; ++k) {
+      QxArgs a;
+      a.pr = c->pr; a.pp = c->pp; a.w = wh.w; a.e = e; a.boys = c->d_boys;
+      const int slot = QX_SLOT_JK(wh.cls, k, fp32 ? 1 : 0);
+      auto fn = ent.jk[k];
+      const long long nq = wh.w.nquart;
+      out.push_back({(double)nq * ent.jk_flops[k], [c, a, fn, fp32, slot, nq](hipStream_t st) {
+                       QxLaunch L{fp32, st};
+                       return timed_launch(c, slot, nq, st, [&] { return fn(&a, &L); });
+                     }});
+    }
   }
 }
 
@@ -1039,10 +1045,15 @@ int qx_profile_slot(int slot, int* kind, int* cls, int* part, int* fp32) {
   int k, c, p = 0, f;
   if (slot < 2 * QX_NUM_CLASSES_MAX) {
     k = 0; f = slot / QX_NUM_CLASSES_MAX; c = slot % QX_NUM_CLASSES_MAX;
-  } else {
+  } else if (slot < QX_SLOT_JK0) {
     const int r = slot - 2 * QX_NUM_CLASSES_MAX;
     k = 1; p = r % QX_DPARTS_CAP;
     const int fc = r / QX_DPARTS_CAP;
+    f = fc / QX_NUM_CLASSES_MAX; c = fc % QX_NUM_CLASSES_MAX;
+  } else {
+    const int r = slot - QX_SLOT_JK0;
+    k = 2; p = r % QX_JKPARTS_CAP;
+    const int fc = r / QX_JKPARTS_CAP;
     f = fc / QX_NUM_CLASSES_MAX; c = fc % QX_NUM_CLASSES_MAX;
   }
   if (kind) *kind = k;
@@ -1087,6 +1098,15 @@ int qx_class_flops(int cls, double* eri, double* parts, int* ncomb) {
     if (parts) parts[k] = e.part_flops[k];
     if (ncomb) ncomb[k] = e.part_ncomb[k];
   }
+  return QX_OK;
+}
+
+int qx_class_jk(int cls, int* nparts, double* flops) {
+  if (cls < 0 || cls >= QX_NUM_CLASSES) return QX_EINVAL;
+  const QxClassEntry& e = qx_class_table[cls];
+  if (nparts) *nparts = e.njk;
+  for (int k = 0; k < e.njk; ++k)
+    if (flops) flops[k] = e.jk_flops[k];
   return QX_OK;
 }
 

@@ -99,6 +99,10 @@ def test_class_table_static_info():
         assert L.qx_class_groups(k, ctypes.byref(ng), gf, gp) == 0
         assert 1 <= ng.value <= npart.value and sum(gp[i] for i in range(ng.value)) == npart.value
         assert abs(sum(gf[i] for i in range(ng.value)) - sum(pf[i] for i in range(npart.value))) < 1e-6 * gf[0] * 48
+        nj = ctypes.c_int()
+        jf = (ctypes.c_double * 8)()
+        assert L.qx_class_jk(k, ctypes.byref(nj), jf) == 0
+        assert 1 <= nj.value <= 6 and all(jf[i] > ef.value / 8 for i in range(nj.value))
     assert len(seen) == 21
 
 
@@ -110,7 +114,7 @@ def test_profile_slots_are_distinct():
         v = [ctypes.c_int() for _ in range(4)]
         assert L.qx_profile_slot(slot, *[ctypes.byref(x) for x in v]) == 0
         key = tuple(x.value for x in v)
-        assert key not in seen and key[0] in (0, 1) and key[3] in (0, 1) and 0 <= key[1] < 21
+        assert key not in seen and key[0] in (0, 1, 2) and key[3] in (0, 1) and 0 <= key[1] < 21
         seen.add(key)
     assert L.qx_profile_slot(L.qx_profile_slots(), None, None, None, None) != 0
 
