@@ -35,6 +35,7 @@ def _function():
         @staticmethod
         def forward(ctx, geom, D, engine, precision, fp32_threshold):
             ctx.engine, ctx.precision, ctx.thr = engine, precision, fp32_threshold
+            ctx.geom_device, ctx.d_device = geom.device, D.device
             Dd = D.detach().to(device=engine.device, dtype=torch.float64).contiguous()
             J, K = engine.jk(Dd, precision=precision, fp32_threshold=fp32_threshold)
             F2 = 2.0 * J - K
@@ -48,9 +49,9 @@ def _function():
             g_geom = g_D = None
             if ctx.needs_input_grad[0]:
                 g = eng.eri_grad(Dd, precision=ctx.precision, fp32_threshold=ctx.thr)
-                g_geom = (gout.to(g.device) * g).reshape(-1, 3)
+                g_geom = (gout.to(g.device) * g).reshape(-1, 3).to(ctx.geom_device)
             if ctx.needs_input_grad[1]:
-                g_D = gout.to(F2.device) * 2.0 * F2
+                g_D = (gout.to(F2.device) * 2.0 * F2).to(ctx.d_device)
             return g_geom, g_D, None, None, None
 
     _EriEnergy.fn = EriEnergy

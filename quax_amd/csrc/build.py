@@ -66,17 +66,21 @@ def _compile(src, force):
     return obj, time.time() - t0, False
 
 
-def build(force=False, jobs=None, lmax=2, fp32=True, verbose=True, budget=None, out=None):
-    """budget: one derivative node budget for every class instead of the
-    tuned per-class table (tuning builds); out: library path."""
+def build(force=False, jobs=None, lmax=2, fp32=True, verbose=True, budget=None, out=None, waves=None):
+    """budget / waves: one derivative node budget / waves-per-SIMD target for
+    every class instead of the tuned per-class table (tuning builds); out:
+    library path."""
     lib = out or LIB
     os.makedirs(OBJ, exist_ok=True)
     os.makedirs(os.path.dirname(os.path.abspath(lib)), exist_ok=True)
     gen_args = [sys.executable, os.path.join(HERE, "gen_eri.py"), GEN, str(lmax)] + ([] if fp32 else ["--no-fp32"])
     env = dict(os.environ)
     env.pop("QX_DERIV_NODES", None)
+    env.pop("QX_GRAD_WAVES", None)
     if budget:
         env["QX_DERIV_NODES"] = str(int(budget))
+    if waves:
+        env["QX_GRAD_WAVES"] = str(int(waves))
     if os.path.isdir(GEN):
         shutil.rmtree(GEN)
     subprocess.run(gen_args, check=True, capture_output=True, env=env)
@@ -110,6 +114,8 @@ if __name__ == "__main__":
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 (mixed-precision) kernels")
     ap.add_argument("--budget", type=int, default=None,
                     help="one derivative node budget for all classes (tuning builds)")
+    ap.add_argument("--waves", type=int, default=None,
+                    help="one waves-per-SIMD target for all adjoint kernels (tuning builds)")
     ap.add_argument("--out", default=None, help="library path (default quax_amd/lib/libquax_hip.so)")
     a = ap.parse_args()
-    build(force=a.force, jobs=a.jobs, lmax=a.lmax, fp32=not a.no_fp32, budget=a.budget, out=a.out)
+    build(force=a.force, jobs=a.jobs, lmax=a.lmax, fp32=not a.no_fp32, budget=a.budget, out=a.out, waves=a.waves)

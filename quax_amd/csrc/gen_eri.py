@@ -43,6 +43,20 @@ if os.path.exists(_BUDGET_FILE):  # written by tune.py from GPU timings
     with open(_BUDGET_FILE) as _fh:
         DERIV_BUDGET = {tuple(int(ch) for ch in k): int(v) for k, v in json.load(_fh)["budget"].items()}
 _BUDGET_OVERRIDE = os.environ.get("QX_DERIV_NODES")
+# waves per SIMD the adjoint kernels of a class are compiled for (0: the
+# compiler's choice, one wave with 512 registers for the heavy classes);
+# from the same tuning file, QX_GRAD_WAVES overrides (tuning builds)
+DERIV_WAVES = {}
+if os.path.exists(_BUDGET_FILE):
+    with open(_BUDGET_FILE) as _fh:
+        DERIV_WAVES = {tuple(int(ch) for ch in k): int(v) for k, v in json.load(_fh).get("waves", {}).items()}
+_WAVES_OVERRIDE = os.environ.get("QX_GRAD_WAVES")
+
+
+def deriv_waves(l):
+    if _WAVES_OVERRIDE:
+        return int(_WAVES_OVERRIDE)
+    return DERIV_WAVES.get(tuple(l), 0)
 
 
 def comps(l):
@@ -549,8 +563,10 @@ def emit_class_files(cls, outdir, fp32):
     groups = part_groups(len(parts))
     for gi, members in enumerate(groups):
         (pi,) = members
+        w = deriv_waves(cls)
+        attr = f" __attribute__((amdgpu_waves_per_eu({w}, {w})))" if w > 0 else ""
         kern = ["template <typename R>",
-                f"__global__ __launch_bounds__(QX_BLOCK) void qx_kg_d{nm}_g{gi}(QxArgs A_) {{",
+                f"__global__ __launch_bounds__(QX_BLOCK){attr} void qx_kg_d{nm}_g{gi}(QxArgs A_) {{",
                 "  __shared__ double sg[QX_GRAD_LDS];",
                 "  qx_grad_body(A_, sg, [&](int P1, int P2, int u1, int u2, double (&g)[QX_GRAD_NCOMB]) {",
                 f"    qx_gr_d{nm}_{pi}<R>(A_, P1, P2, u1, u2, g);", "  });", "}",

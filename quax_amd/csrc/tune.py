@@ -1,5 +1,7 @@
 """Derivative-part autotuner: picks, per angular-momentum class, the node
-budget at which gen_eri.py splits the adjoint-gradient kernel.
+budget at which gen_eri.py splits the adjoint-gradient kernel (--key budget)
+or the waves per SIMD its kernels are compiled for (--key waves; a library
+built with --waves W, W = 0 for the compiler's choice).
 
 A smaller budget splits a class into more parts: each part recomputes the
 recurrence nodes it shares with the others (more FP64 work) but keeps its
@@ -29,6 +31,7 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 
 
 def budget_of(lib_path: str) -> int:
+    """The tuned value a library was built with: the number in its directory name."""
     m = re.search(r"(\d+)", os.path.basename(os.path.dirname(os.path.abspath(lib_path))))
     if not m:
         raise SystemExit(f"cannot read the budget from the directory name of {lib_path}")
@@ -52,13 +55,15 @@ def main():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--timeout", type=int, default=300)
     ap.add_argument("--out", default="deriv_budget.json")
+    ap.add_argument("--key", default="budget", choices=["budget", "waves"])
+    ap.add_argument("--merge", default=None, help="existing tuning JSON whose other keys are kept")
     a = ap.parse_args()
     per = {}
     for lib in a.libs:
         b = budget_of(lib)
         res = measure(lib, a.workload, a.steps, a.timeout)
         per[b] = {cls: t.get("deriv64", 0.0) for cls, t in res["class_kernel_ms_per_step"].items()}
-        print(f"[tune] budget {b}: adjoint kernels {sum(per[b].values()):.2f} ms per step", flush=True)
+        print(f"[tune] {a.key} {b}: adjoint kernels {sum(per[b].values()):.2f} ms per step", flush=True)
     classes = sorted(set().union(*[set(v) for v in per.values()]))
     choice, table = {}, {}
     for cls in classes:
@@ -68,8 +73,15 @@ def main():
         table[cls] = times
         print(f"[tune] {cls}: " + "  ".join(f"{b}:{times[b]:.3f}" for b in sorted(times)) + f"  -> {best}",
               flush=True)
+    out = {}
+    if a.merge and os.path.exists(a.merge):
+        with open(a.merge) as fh:
+            out = json.load(fh)
+    out[a.key] = choice
+    out[f"ms_per_step_{a.key}"] = table
+    out["workload"] = a.workload
     with open(a.out, "w") as fh:
-        json.dump({"budget": choice, "ms_per_step": table, "workload": a.workload}, fh, indent=1, sort_keys=True)
+        json.dump(out, fh, indent=1, sort_keys=True)
     print(f"[tune] wrote {a.out}")
 
 
