@@ -57,12 +57,12 @@ def kernel_table(eng):
             npart = ctypes.c_int()
             L.qx_class_info(c, l4, ctypes.byref(npart))
             ef = ctypes.c_double()
-            pf = (ctypes.c_double * 48)()
-            pc = (ctypes.c_int * 48)()
+            pf = (ctypes.c_double * L.qx_max_parts())()
+            pc = (ctypes.c_int * L.qx_max_parts())()
             L.qx_class_flops(c, ctypes.byref(ef), pf, pc)
             ngr = ctypes.c_int()
-            gf = (ctypes.c_double * 48)()
-            gp = (ctypes.c_int * 48)()
+            gf = (ctypes.c_double * L.qx_max_parts())()
+            gp = (ctypes.c_int * L.qx_max_parts())()
             L.qx_class_groups(c, ctypes.byref(ngr), gf, gp)
             njk = ctypes.c_int()
             jf = (ctypes.c_double * 8)()

@@ -162,12 +162,15 @@ int qx_class_quartets(const qx_ctx* ctx, long long* eri, long long* deri);
 /* Algorithmic FP operations (mul/add 1, fma 2) of one primitive-quartet ERI
  * body of class cls (J/K epilogue excluded) and of each derivative part
  * (adjoint-gradient epilogue included), and the number of derivative blocks
- * each part produces.  parts / ncomb hold up to 48 entries. */
+ * each part produces.  parts / ncomb hold qx_max_parts() entries. */
 int qx_class_flops(int cls, double* eri, double* parts, int* ncomb);
+/* Most derivative parts (and part groups) one class can have: the size of
+ * the per-part arrays of qx_class_flops and qx_class_groups. */
+int qx_max_parts(void);
 /* The adjoint-gradient pass launches one kernel per group of derivative
  * parts of a class (blockIdx.y = part): number of groups, FP operations per
  * primitive quartet of each group (sum over its parts) and parts per group;
- * arrays of up to 48 entries.  Derivative profile slots of the gradient
+ * arrays of qx_max_parts() entries.  Derivative profile slots of the gradient
  * pass are indexed by group. */
 int qx_class_groups(int cls, int* ngroups, double* flops, int* nparts);
 /* The J/K pass launches one kernel per range of a components of a class:

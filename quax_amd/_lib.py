@@ -65,6 +65,7 @@ SIGNATURES = {
     "qx_class_flops": (_i, [_i, _dp, _dp, _ip]),
     "qx_class_groups": (_i, [_i, _ip, _dp, _ip]),
     "qx_class_jk": (_i, [_i, _ip, _dp]),
+    "qx_max_parts": (_i, []),
     "qx_profile_enable": (_i, [_vp, _i]),
     "qx_profile_read": (_i, [_vp, _dp, ctypes.POINTER(_ll), ctypes.POINTER(_ll)]),
 }

@@ -66,7 +66,8 @@ def _compile(src, force):
     return obj, time.time() - t0, False
 
 
-def build(force=False, jobs=None, lmax=2, fp32=True, verbose=True, budget=None, out=None, waves=None):
+def build(force=False, jobs=None, lmax=2, fp32=True, verbose=True, budget=None, out=None, waves=None,
+          csplit=None):
     """budget / waves: one derivative node budget / waves-per-SIMD target for
     every class instead of the tuned per-class table (tuning builds); out:
     library path."""
@@ -77,6 +78,9 @@ def build(force=False, jobs=None, lmax=2, fp32=True, verbose=True, budget=None, 
     env = dict(os.environ)
     env.pop("QX_DERIV_NODES", None)
     env.pop("QX_GRAD_WAVES", None)
+    env.pop("QX_CSPLIT", None)
+    if csplit:
+        env["QX_CSPLIT"] = str(int(csplit))
     if budget:
         env["QX_DERIV_NODES"] = str(int(budget))
     if waves:
@@ -116,6 +120,9 @@ if __name__ == "__main__":
                     help="one derivative node budget for all classes (tuning builds)")
     ap.add_argument("--waves", type=int, default=None,
                     help="one waves-per-SIMD target for all adjoint kernels (tuning builds)")
+    ap.add_argument("--csplit", type=int, default=None,
+                    help="c-component ranges per (a, b) derivative part for every class (tuning builds)")
     ap.add_argument("--out", default=None, help="library path (default quax_amd/lib/libquax_hip.so)")
     a = ap.parse_args()
-    build(force=a.force, jobs=a.jobs, lmax=a.lmax, fp32=not a.no_fp32, budget=a.budget, out=a.out, waves=a.waves)
+    build(force=a.force, jobs=a.jobs, lmax=a.lmax, fp32=not a.no_fp32, budget=a.budget, out=a.out, waves=a.waves,
+          csplit=a.csplit)

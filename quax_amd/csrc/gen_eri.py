@@ -16,9 +16,9 @@ the epilogue), as straight-line code:
 Derivative kernels build d/dX_i (ab|cd) from d/dA_i phi_a = 2 alpha
 phi_{a+1_i} - a_i phi_{a-1_i}.  Every kernel forms d/dA, d/dB, d/dC and
 infers d/dD by translational invariance.  A class whose body would exceed
-DERIV_NODES recurrence nodes is split into parts over the Cartesian
-components of a (then of b): each part evaluates only the output functions
-(a, b, c, d) with a (b) in its range.  A part recomputes the recurrence
+its node budget is split into parts over the Cartesian components of a
+(then of b, then of c): each part evaluates only the output functions
+(a, b, c, d) in its ranges.  A part recomputes the recurrence
 nodes it shares with the others, but its live set fits the register file
 (the unsplit heavy bodies spill several KB per lane to scratch, and scratch
 latency, not arithmetic, then bounds them).  Epilogues: full-tensor scatter
@@ -51,6 +51,21 @@ if os.path.exists(_BUDGET_FILE):
     with open(_BUDGET_FILE) as _fh:
         DERIV_WAVES = {tuple(int(ch) for ch in k): int(v) for k, v in json.load(_fh).get("waves", {}).items()}
 _WAVES_OVERRIDE = os.environ.get("QX_GRAD_WAVES")
+
+
+DERIV_CSPLIT = {}
+if os.path.exists(_BUDGET_FILE):
+    with open(_BUDGET_FILE) as _fh:
+        DERIV_CSPLIT = {tuple(int(ch) for ch in k): int(v) for k, v in json.load(_fh).get("csplit", {}).items()}
+_CSPLIT_OVERRIDE = os.environ.get("QX_CSPLIT")
+
+
+def deriv_csplit(l):
+    """c-component ranges per (a, b) part for classes whose (a, b) parts
+    still exceed their budget (1 = no further split)."""
+    if _CSPLIT_OVERRIDE:
+        return int(_CSPLIT_OVERRIDE)
+    return DERIV_CSPLIT.get(tuple(l), 1)
 
 
 def deriv_waves(l):
@@ -230,10 +245,11 @@ def deriv_targets(l, part):
     la, lb, lc, ld = l
     a0, a1 = part["a"]
     b0, b1 = part["b"]
+    c0, c1 = part.get("c", (0, ncart(lc)))
     T = []
     for a in comps(la)[a0:a1]:
         for b in comps(lb)[b0:b1]:
-            for c in comps(lc):
+            for c in comps(lc)[c0:c1]:
                 for d in comps(ld):
                     for (cen, x) in part["combos"]:
                         q = [a, b, c, d]
@@ -280,16 +296,19 @@ def _ranges(n, k):
 def deriv_parts(l, budget=None):
     """Split the derivative of class l into kernels of at most `budget`
     recurrence nodes: whole class, else contiguous ranges of a components,
-    else one a component with ranges of b components (one (a, b) pair per
-    part is the finest split)."""
+    else one a component with ranges of b components, else one (a, b) pair
+    with ranges of c components (one (a, b, c) triple per part is the finest
+    split)."""
     if budget is None:
         budget = int(_BUDGET_OVERRIDE) if _BUDGET_OVERRIDE else DERIV_BUDGET.get(tuple(l), DERIV_NODES)
     la, lb = l[0], l[1]
     NA, NB = ncart(la), ncart(lb)
     combos = [(c, x) for c in "ABC" for x in range(3)]
 
-    def part(ar, br):
-        return {"combos": combos, "dinv": True, "a": ar, "b": br}
+    NC = ncart(l[2])
+
+    def part(ar, br, cr=(0, NC)):
+        return {"combos": combos, "dinv": True, "a": ar, "b": br, "c": cr}
 
     def nodes(p):
         h, v, _, _, _ = build_dag(deriv_targets(l, p))
@@ -304,9 +323,12 @@ def deriv_parts(l, budget=None):
             return ps
     for k in range(1, NB + 1):
         ps = [part((a, a + 1), r) for a in range(NA) for r in _ranges(NB, k)]
-        if k == NB or all(nodes(p) <= budget for p in ps):
+        if all(nodes(p) <= budget for p in ps):
             return ps
-    raise AssertionError("unreachable")
+    # one (a, b) pair per part is still above the budget: c ranges, as many
+    # as the class's csplit allows (1 keeps the (a, b) parts)
+    kc = min(deriv_csplit(l), NC)
+    return [part((a, a + 1), (b, b + 1), r) for a in range(NA) for b in range(NB) for r in _ranges(NC, kc)]
 
 
 def emit_setup(L, Lt, deriv_w):
@@ -426,6 +448,7 @@ def emit_deriv_core(l, pi, part):
     combos, dinv = part["combos"], part["dinv"]
     a0, a1 = part["a"]
     b0, b1 = part["b"]
+    c0, c1 = part.get("c", (0, ncart(lc)))
     T = deriv_targets(l, part)
     h, v, vn, outs, pos = build_dag(T)
     Lt = sum(l) + 1
@@ -438,10 +461,10 @@ def emit_deriv_core(l, pi, part):
     ti = 0
     for ia, a in enumerate(comps(la)[a0:a1], a0):
         for ib, b in enumerate(comps(lb)[b0:b1], b0):
-            for ic, c in enumerate(comps(lc)):
+            for ic, c in enumerate(comps(lc)[c0:c1], c0):
                 for id_, d in enumerate(comps(ld)):
                     e = ((ia * NB + ib) * NC_ + ic) * ND + id_
-                    le = (((ia - a0) * (b1 - b0) + (ib - b0)) * NC_ + ic) * ND + id_
+                    le = ((((ia - a0) * (b1 - b0) + (ib - b0)) * (c1 - c0)) + (ic - c0)) * ND + id_
                     q = [a, b, c, d]
                     exprs = []
                     last = 0
@@ -488,10 +511,10 @@ def emit_deriv_core(l, pi, part):
     cmask = sum(1 << CENTRES.index(c) for c in set(cc for cc, _ in combos))
     xmask = sum(1 << x for x in set(x for _, x in combos))
     nel = len(elems)
-    stats = dict(part=pi, a=part["a"], b=part["b"], dinv=dinv, vrr=len(v.lines), vrr_flops=v.nflop,
+    stats = dict(part=pi, a=part["a"], b=part["b"], c=(c0, c1), dinv=dinv, vrr=len(v.lines), vrr_flops=v.nflop,
                  hrr=len(h.lines), hrr_flops=h.nflop, L=Lt, N=N, nel=nel, ncomb=NC,
                  flops=core_flops(Lt, v.nflop, h.nflop) + nel * (8 + 5 * NC))
-    return "\n".join(L), "\n".join(G), stats, (cmask, xmask, dinv, NC, a0, a1, b0, b1)
+    return "\n".join(L), "\n".join(G), stats, (cmask, xmask, dinv, NC, a0, a1, b0, b1, c0, c1)
 
 
 def core_flops(L, vrr_flops, hrr_flops):
@@ -554,10 +577,11 @@ def emit_class_files(cls, outdir, fp32):
     parts = deriv_parts(cls)
     gsrc = []
     for pi, part in enumerate(parts):
-        src_d, src_g, st, (cmask, xmask, dinv, NC, a0, a1, b0, b1) = emit_deriv_core(cls, pi, part)
+        src_d, src_g, st, (cmask, xmask, dinv, NC, a0, a1, b0, b1, c0, c1) = emit_deriv_core(cls, pi, part)
         assert (cmask, xmask, dinv, NC) == (7, 7, True, 9)
         args = f"{N}, {dims}, {cmask}, {xmask}, {1 if dinv else 0}, {NC}"
-        write(f"cls_{nm}_d{pi}.hip", [src_d, "", f"QX_DEFINE_DERI_KERNEL(d{nm}_{pi}, {args}, {a0}, {a1}, {b0}, {b1})"])
+        write(f"cls_{nm}_d{pi}.hip", [src_d, "", f"QX_DEFINE_DERI_KERNEL(d{nm}_{pi}, {args}, {a0}, {a1}, {b0}, {b1}, "
+                                                  f"{c0}, {c1})"])
         gsrc.append(src_g)
         stats.append((cls, f"deriv{pi}", st))
     groups = part_groups(len(parts))

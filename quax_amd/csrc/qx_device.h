@@ -322,18 +322,19 @@ __device__ __forceinline__ bool qx_single_atom(const QxArgs& A_, int P1, int P2)
   return a == A_.pr.atom_b[P1] && a == A_.pr.atom_a[P2] && a == A_.pr.atom_b[P2];
 }
 
-// A part covers the output functions with a in [A0, A1) and b in [B0, B1):
-// dv holds NCOMB blocks of that part's NEL elements, in (a, b, c, d) order.
-template <int NA, int NB, int NC, int ND, int A0, int A1, int B0, int B1>
+// A part covers the output functions with a in [A0, A1), b in [B0, B1) and c
+// in [C0, C1): dv holds NCOMB blocks of that part's NEL elements, in
+// (a, b, c, d) order.
+template <int NA, int NB, int NC, int ND, int A0, int A1, int B0, int B1, int C0, int C1>
 struct QxPartShape {
-  static constexpr int NEL = (A1 - A0) * (B1 - B0) * NC * ND;
+  static constexpr int NEL = (A1 - A0) * (B1 - B0) * (C1 - C0) * ND;
 };
 
 template <int NA, int NB, int NC, int ND, int CMASK, int XMASK, int DINV, int NCOMB, int A0, int A1, int B0, int B1,
-          typename R>
-__device__ __forceinline__ void qx_epi_deri(const QxArgs& A_, int P1, int P2,
-                                            const R (&dv)[NCOMB * QxPartShape<NA, NB, NC, ND, A0, A1, B0, B1>::NEL]) {
-  constexpr int NEL = QxPartShape<NA, NB, NC, ND, A0, A1, B0, B1>::NEL;
+          int C0, int C1, typename R>
+__device__ __forceinline__ void qx_epi_deri(
+    const QxArgs& A_, int P1, int P2, const R (&dv)[NCOMB * QxPartShape<NA, NB, NC, ND, A0, A1, B0, B1, C0, C1>::NEL]) {
+  constexpr int NEL = QxPartShape<NA, NB, NC, ND, A0, A1, B0, B1, C0, C1>::NEL;
   const int at[4] = {A_.pr.atom_a[P1], A_.pr.atom_b[P1], A_.pr.atom_a[P2], A_.pr.atom_b[P2]};
   const int n = A_.e.nbf;
   const int ia = A_.pr.bfa[P1], ib = A_.pr.bfb[P1], ic = A_.pr.bfa[P2], id = A_.pr.bfb[P2];
@@ -347,10 +348,10 @@ __device__ __forceinline__ void qx_epi_deri(const QxArgs& A_, int P1, int P2,
 #pragma unroll
       for (int j = B0; j < B1; ++j)
 #pragma unroll
-        for (int k = 0; k < NC; ++k)
+        for (int k = C0; k < C1; ++k)
 #pragma unroll
           for (int l = 0; l < ND; ++l) {
-            const int e = (((i - A0) * (B1 - B0) + (j - B0)) * NC + k) * ND + l;
+            const int e = ((((i - A0) * (B1 - B0) + (j - B0)) * (C1 - C0)) + (k - C0)) * ND + l;
             double dsum[3] = {0.0, 0.0, 0.0};
             int ci = 0;
 #pragma unroll
@@ -475,18 +476,19 @@ __device__ __forceinline__ void qx_epi_deri(const QxArgs& A_, int P1, int P2,
   }
 
 // Array-form derivative kernel: full derivative-tensor scatter.
-#define QX_DEFINE_DERI_KERNEL(NAME, N, NA, NB, NC, ND, CMASK, XMASK, DINV, NCOMB, A0, A1, B0, B1) \
+#define QX_DEFINE_DERI_KERNEL(NAME, N, NA, NB, NC, ND, CMASK, XMASK, DINV, NCOMB, A0, A1, B0, B1, C0, C1) \
   __global__ __launch_bounds__(QX_BLOCK) void qx_k_##NAME(QxArgs A_) {                            \
     const long long t = (long long)blockIdx.x * QX_BLOCK + threadIdx.x;                          \
     if (t >= A_.w.nquart) return;                                                                 \
     int u1, u2, P1, P2;                                                                           \
     qx_locate(A_, t, u1, u2, P1, P2);                                                             \
     if (qx_single_atom(A_, P1, P2)) return;                                                       \
-    static_assert(NCOMB * QxPartShape<NA, NB, NC, ND, A0, A1, B0, B1>::NEL * 8 <= 16384,           \
+    using Shape = QxPartShape<NA, NB, NC, ND, A0, A1, B0, B1, C0, C1>;                            \
+    static_assert(NCOMB * Shape::NEL * 8 <= 16384,                                                \
                   "per-lane derivative block must stay small (scratch is allocated per lane)");    \
-    double dv[NCOMB * QxPartShape<NA, NB, NC, ND, A0, A1, B0, B1>::NEL];                          \
+    double dv[NCOMB * Shape::NEL];                                                                \
     qx_core_##NAME<double>(A_, P1, P2, u1, u2, dv);                                               \
-    qx_epi_deri<NA, NB, NC, ND, CMASK, XMASK, DINV, NCOMB, A0, A1, B0, B1, double>(A_, P1, P2, dv); \
+    qx_epi_deri<NA, NB, NC, ND, CMASK, XMASK, DINV, NCOMB, A0, A1, B0, B1, C0, C1, double>(A_, P1, P2, dv); \
   }                                                                                               \
   extern "C" int qx_launch_##NAME(const QxArgs* a, const QxLaunch* L) {                           \
     if (a->w.nquart <= 0) return 0;                                                               \
@@ -553,7 +555,7 @@ __device__ __forceinline__ void qx_grad_body(const QxArgs& A_, double* sg, Part 
     return (int)hipGetLastError();                                                                \
   }
 
-#define QX_DPARTS_CAP 48  // most derivative parts one class may be split into
+#define QX_DPARTS_CAP 128  // most derivative parts one class may be split into
 #define QX_JKPARTS_CAP 8  // most J/K parts (ranges of a components) of one class
 
 struct QxClassEntry {

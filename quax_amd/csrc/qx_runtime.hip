@@ -1101,6 +1101,8 @@ int qx_class_flops(int cls, double* eri, double* parts, int* ncomb) {
   return QX_OK;
 }
 
+int qx_max_parts(void) { return QX_DPARTS_CAP; }
+
 int qx_class_jk(int cls, int* nparts, double* flops) {
   if (cls < 0 || cls >= QX_NUM_CLASSES) return QX_EINVAL;
   const QxClassEntry& e = qx_class_table[cls];

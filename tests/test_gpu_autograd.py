@@ -50,7 +50,7 @@ def test_backward_matches_finite_differences(gpu, case):
         gD = Dt.grad.cpu().numpy()
         J, K = eng.jk(D)
         E0 = float(torch.sum(torch.as_tensor(D, device="cuda") * (2 * J - K)))
-    assert abs(float(E) - E0) < 1e-12 * max(1.0, abs(E0))
+    assert abs(float(E.detach()) - E0) < 1e-12 * max(1.0, abs(E0))
     # geometry: central differences, every coordinate (basis built per geometry)
     h = 1e-4
     fd = np.zeros_like(g)
