@@ -57,6 +57,19 @@ int qx_initialize(int natom, const double* xyz_bohr, const double* nuclear_charg
 /* Replaces finalize() (libint_interface.cc:97-99). */
 int qx_finalize(qx_ctx* ctx);
 
+/* Batched form (no reference counterpart; Quax runs one molecule per
+ * initialize): as qx_initialize, plus a fragment id per atom.  Shell pairs
+ * and quartets never span two fragments, so one context -- and one kernel
+ * launch per class -- evaluates the independent molecules of a batch
+ * side by side; J/K, gradients and tensors are block diagonal over
+ * fragments.  One-electron integrals (qx_oei*) still couple all nuclei:
+ * take them from a context per molecule.  The Schwarz cut tol^2/max Q uses
+ * the largest Q over the whole batch. */
+int qx_initialize_fragments(int natom, const double* xyz_bohr, const double* nuclear_charges, int nshell,
+                            const int* shell_l, const int* shell_atom, const int* shell_nprim, const double* exps,
+                            const double* coefs, int pure, double ints_tol, int device, const int* atom_fragment,
+                            qx_ctx** out);
+
 const char* qx_last_error(const qx_ctx* ctx);
 
 /* Basis sizes: AO functions (spherical count when pure) and Cartesian. */

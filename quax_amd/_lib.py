@@ -38,6 +38,8 @@ _ll = ctypes.c_longlong
 SIGNATURES = {
     "qx_initialize": (_i, [_i, _dp, _dp, _i, _ip, _ip, _ip, _dp, _dp, _i, ctypes.c_double, _i,
                            ctypes.POINTER(_vp)]),
+    "qx_initialize_fragments": (_i, [_i, _dp, _dp, _i, _ip, _ip, _ip, _dp, _dp, _i, ctypes.c_double, _i, _ip,
+                                     ctypes.POINTER(_vp)]),
     "qx_finalize": (_i, [_vp]),
     "qx_last_error": (ctypes.c_char_p, [_vp]),
     "qx_nbf": (_i, [_vp, _ip, _ip]),

@@ -45,6 +45,10 @@ struct qx_ctx {
   int pure = 0, nbf = 0, ncart = 0;
   double tol = 1e-12;
   bool k1 = true;          // every shell has one primitive
+  // fragment of each atom: shell pairs and quartets never mix fragments, so
+  // one context (and one launch per class) serves a batch of molecules
+  std::vector<int> frag;
+  bool fragmented = false;
   std::string err;
 
   // device memory owned by the context
@@ -60,11 +64,13 @@ struct qx_ctx {
   // host copies of the sorted screened pairs
   std::vector<int> h_pair_sa, h_pair_sb, h_pair_class, h_pp_first, h_pp_n;
   std::vector<double> h_pair_Q;
+  std::vector<int> h_pair_frag;
   std::vector<int> class_pair_off, class_pair_cnt;   // per pair class, in sorted pairs
   std::vector<int> class_pp_off, class_pp_cnt;       // per pair class, in pseudo pairs
   // quartet lists (one per canonical class)
   std::vector<QxWorkHost> work;
   long long* d_prefix = nullptr;
+  int* d_kstart = nullptr;               // per bra pseudo pair, fragmented contexts only
   long long nquart_total = 0;            // canonical contracted quartets
   long long npq_total = 0;               // pseudo-pair quartets
   long long store_total = 0;

@@ -271,12 +271,38 @@ void check_launch(int rc, const char* what) {
 
 int class_index(int X, int Y) { return X * (X + 1) / 2 + Y; }
 
+// Positions [ss, se) of fragment f within the class list that starts at
+// sorted pair ko and holds kn pairs (sorted by fragment, then Schwarz value).
+void frag_segment(const qx_ctx* c, int ko, int kn, int f, int& ss, int& se) {
+  const int* fr = c->h_pair_frag.data() + ko;
+  ss = (int)(std::lower_bound(fr, fr + kn, f) - fr);
+  se = (int)(std::upper_bound(fr, fr + kn, f) - fr);
+}
+
+// Admissible kets of bra pair i (position in class X) in class Y: positions
+// [ss, ss + kc) of Y's list, i.e. the bra's fragment, Q_i Q_j >= bound (a
+// prefix of the descending segment) and canonical j <= i when X == Y.
+void ket_window(const qx_ctx* c, int X, int Y, int i, double bound, int& ss, int& kc) {
+  const int bo = c->class_pair_off[X], ko = c->class_pair_off[Y], kn = c->class_pair_cnt[Y];
+  int se;
+  frag_segment(c, ko, kn, c->h_pair_frag[bo + i], ss, se);
+  const double Qi = c->h_pair_Q[bo + i];
+  int lo = ss, hi = se;
+  while (lo < hi) {
+    const int mid = (lo + hi) / 2;
+    if (Qi * c->h_pair_Q[ko + mid] >= bound) lo = mid + 1; else hi = mid;
+  }
+  kc = lo - ss;
+  if (X == Y) kc = std::max(0, std::min(kc, i - ss + 1));
+}
+
 void schwarz_and_lists(qx_ctx* c) {
   const int ns = c->nshell;
   // ---- every shell pair, class-sorted, for the Schwarz pass
   std::vector<PairRec> all;
   for (int s1 = 0; s1 < ns; ++s1)
     for (int s2 = 0; s2 <= s1; ++s2) {
+      if (c->frag[c->atom[s1]] != c->frag[c->atom[s2]]) continue;  // pairs never span fragments
       PairRec r;
       r.s1 = s1; r.s2 = s2;
       if (c->l[s1] >= c->l[s2]) { r.a = s1; r.b = s2; } else { r.a = s2; r.b = s1; }
@@ -355,8 +381,10 @@ void schwarz_and_lists(qx_ctx* c) {
   std::vector<PairRec> scr;
   for (auto& r : all)
     if (r.Q >= cut) scr.push_back(r);
-  std::sort(scr.begin(), scr.end(), [](const PairRec& x, const PairRec& y) {
+  std::sort(scr.begin(), scr.end(), [c](const PairRec& x, const PairRec& y) {
     if (x.cls != y.cls) return x.cls < y.cls;
+    const int fx = c->frag[c->atom[x.a]], fy = c->frag[c->atom[y.a]];
+    if (fx != fy) return fx < fy;
     if (x.Q != y.Q) return x.Q > y.Q;
     if (x.s1 != y.s1) return x.s1 < y.s1;
     return x.s2 < y.s2;
@@ -364,10 +392,11 @@ void schwarz_and_lists(qx_ctx* c) {
   build_pair_arrays(c, scr, c->pr, c->pp, c->npseudo, &c->h_pp_first, &c->h_pp_n);
   c->npairs = (int)scr.size();
   c->h_pair_sa.resize(c->npairs); c->h_pair_sb.resize(c->npairs);
-  c->h_pair_class.resize(c->npairs); c->h_pair_Q.resize(c->npairs);
+  c->h_pair_class.resize(c->npairs); c->h_pair_Q.resize(c->npairs); c->h_pair_frag.resize(c->npairs);
   for (int i = 0; i < c->npairs; ++i) {
     c->h_pair_sa[i] = scr[i].a; c->h_pair_sb[i] = scr[i].b;
     c->h_pair_class[i] = scr[i].cls; c->h_pair_Q[i] = scr[i].Q;
+    c->h_pair_frag[i] = c->frag[c->atom[scr[i].a]];
   }
   c->class_pair_off.assign(nclass, 0); c->class_pair_cnt.assign(nclass, 0);
   for (int i = c->npairs - 1; i >= 0; --i) c->class_pair_off[scr[i].cls] = i;
@@ -386,6 +415,8 @@ void schwarz_and_lists(qx_ctx* c) {
       if (A == c->atom[c->h_pair_sb[P]]) intra[X][A].push_back(i);
     }
   std::vector<long long> allpre;
+  std::vector<int> allkst;  // per bra pseudo pair: first ket (fragmented contexts)
+  std::vector<long long> kst_off;
   c->nquart_total = 0; c->npq_total = 0; c->store_total = 0;
   for (int X = 0; X < nclass; ++X) {
     for (int Y = 0; Y <= X; ++Y) {
@@ -402,17 +433,13 @@ void schwarz_and_lists(qx_ctx* c) {
       const int u0 = c->h_pp_first[bo];
       const int u1 = c->h_pp_first[bo + bn - 1] + c->h_pp_n[bo + bn - 1];
       allpre.push_back(0);
+      kst_off.push_back((long long)allkst.size());
       long long run = 0, ncq = 0;
       for (int i = 0; i < bn; ++i) {
-        const double Qi = c->h_pair_Q[bo + i];
         // kets with Q_i Q_j >= tol^2 form a prefix of the descending list
-        int lo = 0, hi = kn;
-        while (lo < hi) {
-          const int mid = (lo + hi) / 2;
-          if (Qi * c->h_pair_Q[ko + mid] >= tol2) lo = mid + 1; else hi = mid;
-        }
-        int kc = lo;
-        if (X == Y) kc = std::min(kc, i + 1);
+        // (of the bra's fragment segment)
+        int ss, kc;
+        ket_window(c, X, Y, i, tol2, ss, kc);
         ncq += kc;
         {
           const int P = bo + i;
@@ -420,13 +447,15 @@ void schwarz_and_lists(qx_ctx* c) {
           long long single = 0;
           if (A == c->atom[c->h_pair_sb[P]]) {
             const auto& v = intra[Y][A];
-            single = std::lower_bound(v.begin(), v.end(), kc) - v.begin();
+            single = (std::lower_bound(v.begin(), v.end(), ss + kc) - v.begin()) -
+                     (std::lower_bound(v.begin(), v.end(), ss) - v.begin());
           }
           c->ndq_class[class_index(X, Y)] += kc - single;
         }
         for (int k = 0; k < c->h_pp_n[bo + i]; ++k) {
-          run += kcum[kc];
+          run += kcum[ss + kc] - kcum[ss];
           allpre.push_back(run);
+          allkst.push_back((int)kcum[ss]);
         }
       }
       wh.w.bra_off = u0; wh.w.nbra = u1 - u0;
@@ -441,12 +470,22 @@ void schwarz_and_lists(qx_ctx* c) {
       c->nquart_total += ncq;
       c->nq_class[class_index(X, Y)] += ncq;
       c->npq_total += run;
-      if (run > 0) c->work.push_back(wh);
-      else allpre.resize(wh.prefix_off);
+      if (run > 0) {
+        c->work.push_back(wh);
+      } else {
+        allpre.resize(wh.prefix_off);
+        allkst.resize(kst_off.back());
+        kst_off.pop_back();
+      }
     }
   }
   c->d_prefix = dupload(c, allpre);
-  for (auto& wh : c->work) wh.w.prefix = c->d_prefix + wh.prefix_off;
+  if (c->fragmented) c->d_kstart = dupload(c, allkst);
+  for (size_t k = 0; k < c->work.size(); ++k) {
+    auto& wh = c->work[k];
+    wh.w.prefix = c->d_prefix + wh.prefix_off;
+    wh.w.kstart = c->fragmented ? c->d_kstart + kst_off[k] : nullptr;
+  }
 }
 
 // Work matrices of a J/K or gradient pass on stream s (see qx_ctx::ws).
@@ -477,16 +516,8 @@ void build_split(qx_ctx* c, double thr) {
   const double tol2 = c->tol * c->tol, thr2 = thr * thr;
   const int nclass = pair_class(QX_LMAX_SUPPORTED, QX_LMAX_SUPPORTED) + 1;
   std::vector<long long> pre64, pre32;
-  std::vector<int> kst;
+  std::vector<int> kst, k64st;  // per bra pseudo pair: first FP32 / first FP64 ket
   std::vector<long long> off64, off32, offk;
-  auto count_ge = [&](double Qi, int ko, int kn, double bound) {
-    int lo = 0, hi = kn;
-    while (lo < hi) {
-      const int mid = (lo + hi) / 2;
-      if (Qi * c->h_pair_Q[ko + mid] >= bound) lo = mid + 1; else hi = mid;
-    }
-    return lo;
-  };
   for (int X = 0; X < nclass; ++X)
     for (int Y = 0; Y <= X; ++Y) {
       const int bo = c->class_pair_off[X], bn = c->class_pair_cnt[X];
@@ -511,18 +542,19 @@ void build_split(qx_ctx* c, double thr) {
       pre32.push_back(0);
       long long run64 = 0, run32 = 0;
       for (int i = 0; i < bn; ++i) {
-        const double Qi = c->h_pair_Q[bo + i];
-        int kc = count_ge(Qi, ko, kn, tol2);
-        if (X == Y) kc = std::min(kc, i + 1);
-        const int k64 = std::min(kc, count_ge(Qi, ko, kn, thr2));
+        int ss, kc, ss2, k64;
+        ket_window(c, X, Y, i, tol2, ss, kc);
+        ket_window(c, X, Y, i, thr2, ss2, k64);
+        k64 = std::min(kc, k64);
         S.nq64 += k64;
         S.nq32 += kc - k64;
         for (int k = 0; k < c->h_pp_n[bo + i]; ++k) {
-          run64 += kcum[k64];
-          run32 += kcum[kc] - kcum[k64];
+          run64 += kcum[ss + k64] - kcum[ss];
+          run32 += kcum[ss + kc] - kcum[ss + k64];
           pre64.push_back(run64);
           pre32.push_back(run32);
-          kst.push_back((int)kcum[k64]);
+          k64st.push_back((int)kcum[ss]);
+          kst.push_back((int)kcum[ss + k64]);
         }
       }
       w.w.nquart = run64;
@@ -540,8 +572,10 @@ void build_split(qx_ctx* c, double thr) {
   const long long* d64 = static_cast<const long long*>(up(pre64.data(), pre64.size() * sizeof(long long)));
   const long long* d32 = static_cast<const long long*>(up(pre32.data(), pre32.size() * sizeof(long long)));
   const int* dk = static_cast<const int*>(up(kst.data(), kst.size() * sizeof(int)));
+  const int* dk64 = static_cast<const int*>(up(k64st.data(), k64st.size() * sizeof(int)));
   for (size_t i = 0; i < S.w64.size(); ++i) {
     S.w64[i].w.prefix = d64 + off64[i];
+    S.w64[i].w.kstart = c->fragmented ? dk64 + offk[i] : nullptr;
     S.w32[i].w.prefix = d32 + off32[i];
     S.w32[i].w.kstart = dk + offk[i];
   }
@@ -596,9 +630,10 @@ thread_local std::string g_init_error;
 // ============================================================== C ABI
 extern "C" {
 
-int qx_initialize(int natom, const double* xyz_bohr, const double* nuclear_charges, int nshell,
-                  const int* shell_l, const int* shell_atom, const int* shell_nprim, const double* exps,
-                  const double* coefs, int pure, double ints_tol, int device, qx_ctx** out) {
+static int init_impl(int natom, const double* xyz_bohr, const double* nuclear_charges, int nshell,
+                     const int* shell_l, const int* shell_atom, const int* shell_nprim, const double* exps,
+                     const double* coefs, int pure, double ints_tol, int device, const int* atom_fragment,
+                     qx_ctx** out) {
   if (!out) return QX_EINVAL;
   *out = nullptr;
   qx_ctx* c = new qx_ctx();
@@ -612,6 +647,14 @@ int qx_initialize(int natom, const double* xyz_bohr, const double* nuclear_charg
     QX_HIP(hipSetDevice(device));
     c->device = device;
     c->natom = natom;
+    c->frag.assign(natom, 0);
+    if (atom_fragment) {
+      for (int a = 0; a < natom; ++a) {
+        QX_REQUIRE(atom_fragment[a] >= 0, QX_EINVAL, "fragment ids must be non-negative");
+        c->frag[a] = atom_fragment[a];
+      }
+      c->fragmented = true;
+    }
     c->xyz.assign(xyz_bohr, xyz_bohr + 3 * natom);
     c->Z.assign(nuclear_charges, nuclear_charges + natom);
     for (double v : c->xyz) QX_REQUIRE(std::isfinite(v), QX_EINVAL, "non-finite coordinate");
@@ -654,6 +697,22 @@ int qx_initialize(int natom, const double* xyz_bohr, const double* nuclear_charg
   }
   *out = c;
   return QX_OK;
+}
+
+int qx_initialize(int natom, const double* xyz_bohr, const double* nuclear_charges, int nshell,
+                  const int* shell_l, const int* shell_atom, const int* shell_nprim, const double* exps,
+                  const double* coefs, int pure, double ints_tol, int device, qx_ctx** out) {
+  return init_impl(natom, xyz_bohr, nuclear_charges, nshell, shell_l, shell_atom, shell_nprim, exps, coefs, pure,
+                   ints_tol, device, nullptr, out);
+}
+
+int qx_initialize_fragments(int natom, const double* xyz_bohr, const double* nuclear_charges, int nshell,
+                            const int* shell_l, const int* shell_atom, const int* shell_nprim, const double* exps,
+                            const double* coefs, int pure, double ints_tol, int device, const int* atom_fragment,
+                            qx_ctx** out) {
+  if (!atom_fragment) return QX_EINVAL;
+  return init_impl(natom, xyz_bohr, nuclear_charges, nshell, shell_l, shell_atom, shell_nprim, exps, coefs, pure,
+                   ints_tol, device, atom_fragment, out);
 }
 
 int qx_finalize(qx_ctx* c) {

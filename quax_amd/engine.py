@@ -23,7 +23,9 @@ def _require_gpu():
 
 
 class EriEngine:
-    def __init__(self, molecule, basis, ints_tol: float = 1e-14, device: int = 0):
+    def __init__(self, molecule, basis, ints_tol: float = 1e-14, device: int = 0, fragments=None):
+        """fragments: optional fragment id per atom (qx_initialize_fragments):
+        no shell pair or quartet spans two fragments."""
         torch = _require_gpu()
         self.torch = torch
         self.L = _lib.load()
@@ -40,10 +42,17 @@ class EriEngine:
         ip = ctypes.POINTER(ctypes.c_int)
         with torch.cuda.device(self.device):
             torch.cuda.current_stream().synchronize()
-            rc = self.L.qx_initialize(self.natom, xyz.ctypes.data_as(dp), Z.ctypes.data_as(dp), len(l),
-                                      l.ctypes.data_as(ip), atom.ctypes.data_as(ip), nprim.ctypes.data_as(ip),
-                                      exps.ctypes.data_as(dp), coefs.ctypes.data_as(dp), 1 if basis.pure else 0,
-                                      float(ints_tol), device, ctypes.byref(ctx))
+            args = (self.natom, xyz.ctypes.data_as(dp), Z.ctypes.data_as(dp), len(l), l.ctypes.data_as(ip),
+                    atom.ctypes.data_as(ip), nprim.ctypes.data_as(ip), exps.ctypes.data_as(dp),
+                    coefs.ctypes.data_as(dp), 1 if basis.pure else 0, float(ints_tol), device)
+            if fragments is None:
+                rc = self.L.qx_initialize(*args, ctypes.byref(ctx))
+            else:
+                fr = np.ascontiguousarray(fragments, dtype=np.int32)
+                if fr.shape != (self.natom,):
+                    raise ValueError("fragments needs one id per atom")
+                self._keep = self._keep + (fr,)
+                rc = self.L.qx_initialize_fragments(*args, fr.ctypes.data_as(ip), ctypes.byref(ctx))
         if rc != 0:
             raise _lib.QuaxNativeError(rc, "qx_initialize: " + self.L.qx_last_error(None).decode())
         self.ctx = ctx

@@ -1,0 +1,93 @@
+"""Batched form: molecules as fragments of one context (one launch per class
+serves them all) against one context per molecule, and the lock-step
+batched RHF against restricted_hartree_fock + rhf_gradient per molecule."""
+import numpy as np
+import pytest
+
+from quax_amd import workloads
+from quax_amd.basis import BasisSet, Shell
+from quax_amd.molecule import Molecule
+
+pytestmark = pytest.mark.gpu
+
+
+def _smallest(k):
+    ms = workloads.batch256()
+    return sorted(ms, key=lambda m: (m.natom(), m.nelectron()))[:k]
+
+
+def _combined(mols, bases):
+    symbols, coords, shells, frag = [], [], [], []
+    aoff = 0
+    for k, (m, b) in enumerate(zip(mols, bases)):
+        symbols += [m.symbol(i) for i in range(m.natom())]
+        coords.append(np.asarray(m.geometry()).reshape(-1, 3))
+        shells += [Shell(s.l, s.atom + aoff, list(s.exps), list(s.coefs)) for s in b.shells]
+        frag += [k] * m.natom()
+        aoff += m.natom()
+    return Molecule(symbols, np.vstack(coords)), BasisSet(bases[0].basis_name, shells, pure=bases[0].pure), frag
+
+
+def test_fragment_context_equals_separate_contexts(gpu):
+    import torch
+    from quax_amd.engine import EriEngine
+    mols = _smallest(3)
+    bases = [BasisSet.build(m, "even-tempered", pure=True) for m in mols]
+    comb, cb, frag = _combined(mols, bases)
+    nb = [b.nbf() for b in bases]
+    off = np.concatenate([[0], np.cumsum(nb)]).astype(int)
+    rng = np.random.default_rng(4)
+    Ds = []
+    for n in nb:
+        X = rng.standard_normal((n, n)) * 0.3
+        Ds.append(X @ X.T / n)
+    D = np.zeros((off[-1], off[-1]))
+    for k, Dk in enumerate(Ds):
+        D[off[k]:off[k + 1], off[k]:off[k + 1]] = Dk
+    with EriEngine(comb, cb, ints_tol=1e-12, fragments=frag) as eng:
+        J, K = (x.cpu().numpy() for x in eng.jk(D))
+        g = eng.eri_grad(D).cpu().numpy()
+        nq = eng.num_quartets()
+    nat = np.concatenate([[0], np.cumsum([m.natom() for m in mols])]).astype(int)
+    total = 0
+    for k, (m, b) in enumerate(zip(mols, bases)):
+        with EriEngine(m, b, ints_tol=1e-12) as e1:
+            Jk, Kk = (x.cpu().numpy() for x in e1.jk(Ds[k]))
+            gk = e1.eri_grad(Ds[k]).cpu().numpy()
+            total += e1.num_quartets()
+        s = slice(off[k], off[k + 1])
+        assert np.abs(J[s, s] - Jk).max() < 1e-11
+        assert np.abs(K[s, s] - Kk).max() < 1e-11
+        assert np.abs(g[3 * nat[k]:3 * nat[k + 1]] - gk).max() < 1e-10
+        # nothing leaks between molecules
+        other = np.ones(off[-1], dtype=bool)
+        other[s] = False
+        assert np.abs(J[s][:, other]).max() == 0.0 and np.abs(K[s][:, other]).max() == 0.0
+    assert nq == total
+
+
+def _well_behaved():
+    """Closed-shell molecules with one clear SCF ground state (random
+    synthetic geometries can have several SCF solutions, which DIIS may
+    reach along different paths)."""
+    return [Molecule(["O", "H", "H"], [[0.0, 0.0, 0.0], [0.0, 1.43, 1.1], [0.0, -1.43, 1.1]]),
+            Molecule(["H", "H"], [[0.0, 0.0, 0.0], [0.0, 0.0, 1.4]]),
+            Molecule(["C", "O"], [[0.0, 0.0, 0.0], [0.0, 0.0, 2.13]])]
+
+
+def test_batched_rhf_matches_single_molecule_runs(gpu):
+    from quax_amd.engine import EriEngine
+    from quax_amd.methods.batch import batched_rhf
+    from quax_amd.methods.hartree_fock import restricted_hartree_fock, rhf_gradient
+    mols = _well_behaved()
+    E, G, aux = batched_rhf(mols, "even-tempered", pure=True, ints_tol=1e-12, conv=1e-10)
+    assert all(aux["converged"])
+    for k, m in enumerate(mols):
+        b = BasisSet.build(m, "even-tempered", pure=True)
+        with EriEngine(m, b, ints_tol=1e-12) as e1:
+            Ek, auxk = restricted_hartree_fock(e1, m.nelectron(), {"diis": True, "spectral_shift": False,
+                                                                  "conv": 1e-10, "maxit": 200},
+                                               return_aux_data=True)
+            gk = rhf_gradient(e1, auxk)
+        assert abs(E[k] - Ek) < 1e-9
+        assert np.abs(G[k] - gk).max() < 1e-7
