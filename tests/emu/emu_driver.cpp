@@ -38,7 +38,21 @@ int main(int argc, char** argv) {
   for (auto& v : co) std::fscanf(f, "%lf", &v);
   std::fscanf(f, "%d %lf", &pure, &tol);
   qx_ctx* c = nullptr;
-  int rc = qx_initialize(natom, xyz.data(), Z.data(), nshell, l.data(), at.data(), np.data(), ex.data(), co.data(),
+  // EMU_FRAGMENTS="f0 f1 ...": one fragment id per atom (qx_initialize_fragments)
+  std::vector<int> frag;
+  if (const char* fs = std::getenv("EMU_FRAGMENTS")) {
+    char* end = nullptr;
+    for (const char* p = fs; *p;) {
+      const long v = std::strtol(p, &end, 10);
+      if (end == p) break;
+      frag.push_back((int)v);
+      p = end;
+    }
+  }
+  int rc = !frag.empty()
+               ? qx_initialize_fragments(natom, xyz.data(), Z.data(), nshell, l.data(), at.data(), np.data(),
+                                         ex.data(), co.data(), pure, tol, 0, frag.data(), &c)
+               : qx_initialize(natom, xyz.data(), Z.data(), nshell, l.data(), at.data(), np.data(), ex.data(), co.data(),
                          pure, tol, 0, &c);
   if (rc) die(nullptr, rc, "initialize");
   int nbf, nc;

@@ -3,7 +3,7 @@ on one molecule and compares every output with the CPU oracle -- test
 infrastructure for checking kernel memory safety and numerics without a GPU.
 
 Usage (from the repository root): python tests/emu/run_emu.py CASE
-CASE: water | small_d | contracted_d | pure_small_d
+CASE: water | small_d | contracted_d | pure_small_d | cluster2 | fragments
 """
 import os
 import subprocess
@@ -111,7 +111,84 @@ def run(name, tol=1e-14, seed=3):
     return errs
 
 
+def _driver(mol, bs, D, tol, env_extra=None):
+    l, atom, nprim, exps, coefs = bs.arrays()
+    lines = [str(mol.natom()), " ".join(repr(float(v)) for v in mol.geometry().reshape(-1)),
+             " ".join(repr(mol.charge(i)) for i in range(mol.natom())), str(len(l))]
+    lines += [f"{l[s]} {atom[s]} {nprim[s]}" for s in range(len(l))]
+    lines += [" ".join(repr(float(v)) for v in exps), " ".join(repr(float(v)) for v in coefs),
+              f"{1 if bs.pure else 0} {tol!r}", " ".join(repr(float(v)) for v in D.reshape(-1))]
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout = os.path.join(td, "in.txt"), os.path.join(td, "out.bin")
+        open(fin, "w").write("\n".join(lines) + "\n")
+        env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=1", **(env_extra or {}))
+        r = subprocess.run([os.path.join(HERE, "_out", "emu_driver"), fin, fout], capture_output=True, text=True,
+                           env=env, timeout=3600)
+        if r.returncode != 0:
+            print(r.stderr[-6000:])
+            raise SystemExit(f"emu_driver failed with {r.returncode}")
+        return np.fromfile(fout)
+
+
+def run_fragments(tol=1e-14, seed=5):
+    """Water, H2 and CO (STO-3G) as three fragments of one context: J, K and
+    the two-electron gradient of a block-diagonal density against the oracle
+    on each molecule alone."""
+    from quax_amd.basis import Shell
+    from quax_amd.molecule import Molecule
+    mols = [workloads.water_sto3g(), Molecule(["H", "H"], [[0, 0, 0], [0, 0, 1.4]]),
+            Molecule(["C", "O"], [[0, 0, 0], [0, 0, 2.13]])]
+    bases = [BasisSet.build(m, "sto-3g") for m in mols]
+    symbols, coords, shells, frag = [], [], [], []
+    aoff = 0
+    for k, (m, b) in enumerate(zip(mols, bases)):
+        symbols += [m.symbol(i) for i in range(m.natom())]
+        coords.append(np.asarray(m.geometry()).reshape(-1, 3))
+        shells += [Shell(s.l, s.atom + aoff, list(s.exps), list(s.coefs)) for s in b.shells]
+        frag += [k] * m.natom()
+        aoff += m.natom()
+    from quax_amd.molecule import Molecule as M
+    comb = M(symbols, np.vstack(coords))
+    cb = BasisSet("sto-3g", shells)
+    nb = [b.nbf() for b in bases]
+    off = np.concatenate([[0], np.cumsum(nb)]).astype(int)
+    rng = np.random.default_rng(seed)
+    D = np.zeros((off[-1], off[-1]))
+    Ds = []
+    for k, n in enumerate(nb):
+        X = rng.standard_normal((n, n)) * 0.3
+        Ds.append(X @ X.T / n)
+        D[off[k]:off[k + 1], off[k]:off[k + 1]] = Ds[-1]
+    out = _driver(comb, cb, D, tol, {"EMU_FRAGMENTS": " ".join(map(str, frag))})
+    n, N = off[-1], comb.natom()
+    n2, n4 = n * n, n ** 4
+    pos = n4 if n <= 60 else 0          # G (not compared: no cross-fragment integrals)
+    pos += 3 * N * n4 if n <= 40 else 0
+    J = out[pos:pos + n2].reshape(n, n); pos += n2
+    K = out[pos:pos + n2].reshape(n, n); pos += n2
+    g = out[pos:pos + 3 * N]
+    errs = {"J": 0.0, "K": 0.0, "g": 0.0, "Jx": 0.0}
+    nat = np.concatenate([[0], np.cumsum([m.natom() for m in mols])]).astype(int)
+    for k, (m, b) in enumerate(zip(mols, bases)):
+        ob = oracle.OracleBasis(m, b)
+        G = ob.compute_2e_int(tol)
+        s_ = slice(off[k], off[k + 1])
+        errs["J"] = max(errs["J"], np.abs(J[s_, s_] - np.einsum("pqrs,rs->pq", G, Ds[k])).max())
+        errs["K"] = max(errs["K"], np.abs(K[s_, s_] - np.einsum("prqs,rs->pq", G, Ds[k])).max())
+        dG = ob.eri_deriv_core1(tol)
+        gk = 2 * np.einsum("xpqrs,pq,rs->x", dG, Ds[k], Ds[k]) - np.einsum("xpqrs,pr,qs->x", dG, Ds[k], Ds[k])
+        errs["g"] = max(errs["g"], np.abs(g[3 * nat[k]:3 * nat[k + 1]] - gk).max())
+        other = np.ones(n, dtype=bool)
+        other[s_] = False
+        errs["Jx"] = max(errs["Jx"], np.abs(J[s_][:, other]).max(), np.abs(K[s_][:, other]).max())
+    return errs
+
+
 if __name__ == "__main__":
+    if sys.argv[1:] == ["fragments"]:
+        e = run_fragments()
+        print("fragments", " ".join(f"{k}={v:.2e}" for k, v in e.items()), flush=True)
+        sys.exit(0)
     for nm in sys.argv[1:] or ["water"]:
         e = run(nm)
         print(nm, " ".join(f"{k}={v:.2e}" for k, v in e.items()), flush=True)
