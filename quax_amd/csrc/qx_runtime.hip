@@ -253,15 +253,20 @@ void run_launches(qx_ctx* c, std::vector<Launch>& v, hipStream_t s, const char* 
   QX_HIP(hipEventRecord(c->fork_ev[0], s));
   for (int k = 0; k < ns - 1; ++k) QX_HIP(hipStreamWaitEvent(c->side[k], c->fork_ev[0], 0));
   std::vector<double> load(ns, 0.0);
+  int rc = 0;
   for (auto& x : v) {
     const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
     load[k] += x.cost;
-    check_launch(x.go(k == 0 ? s : c->side[k - 1]), what);
+    rc = x.go(k == 0 ? s : c->side[k - 1]);
+    if (rc != 0) break;
   }
+  // join even after a failed launch, so the caller's stream never runs
+  // ahead of work already queued on the side streams
   for (int k = 0; k < ns - 1; ++k) {
     QX_HIP(hipEventRecord(c->fork_ev[k + 1], c->side[k]));
     QX_HIP(hipStreamWaitEvent(s, c->fork_ev[k + 1], 0));
   }
+  check_launch(rc, what);
 }
 
 void check_launch(int rc, const char* what) {

@@ -35,9 +35,23 @@ DEFAULT_OPTIONS = {
     "precision": "fp64",          # or "mixed"
     "fp32_threshold": 1.0e-8,
     "conv": None,                 # override the reference's 1e-9 / 1e-10
+    "eigh": "auto",               # Fock diagonalisation: "device", "host" or "auto" (host up to 512)
     "pure": False,
     "device": 0,
 }
+
+
+def _eigh(Fp, where="auto"):
+    """Symmetric eigensolver of the orthogonalised Fock matrix, FP64.  For
+    n <= 512 LAPACK on the host is faster than the device solver (rocSOLVER
+    issues ~1000 small kernels for n = 280: ~6.4 ms against ~3.7 ms plus two
+    0.6 MB copies); larger matrices stay on the device."""
+    import torch
+    host = where == "host" or (where == "auto" and Fp.shape[-1] <= 512)
+    if not host:
+        return torch.linalg.eigh(Fp)
+    w, v = np.linalg.eigh(Fp.detach().cpu().numpy())
+    return (torch.as_tensor(w, dtype=Fp.dtype).to(Fp.device), torch.as_tensor(v, dtype=Fp.dtype).to(Fp.device))
 
 
 def check_options(options):
@@ -107,7 +121,7 @@ def restricted_hartree_fock(engine, nelectrons, options=None, return_aux_data=Fa
     def rhf_iter(F, D):
         E = torch.sum((F + H) * D) + Enuc
         Fp = A.T @ F @ A + shift
-        eps, C2 = torch.linalg.eigh(Fp)
+        eps, C2 = _eigh(Fp, opts["eigh"])
         C = A @ C2
         Cocc = C[:, :ndocc]
         return E, Cocc @ Cocc.T, C, eps
