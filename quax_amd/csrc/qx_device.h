@@ -406,10 +406,11 @@ __device__ __forceinline__ void qx_epi_deri(
 
 // K' rows staged in LDS when a block's quartets share one bra pair: for
 // classes with at least QX_JK_ROWS_MIN K' terms per quartet and rows of at
-// most QX_JK_LDS_MAX bytes.  The block's K' updates then cost LDS atomics
-// plus one coalesced pass of global atomics over the touched row entries.
+// most QX_JK_LDS_MAX bytes (gfx950 gives a workgroup up to 160 KB of LDS).
+// The block's K' updates then cost LDS atomics plus one coalesced pass of
+// global atomics over the touched row entries.
 #define QX_JK_ROWS_MIN 4
-#define QX_JK_LDS_MAX 65536
+#define QX_JK_LDS_MAX (160 * 1024)
 
 // Streaming J/K kernel: every integral is contracted with the density as
 // soon as it is formed; the per-thread J'/K' partials are flushed with
