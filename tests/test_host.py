@@ -177,3 +177,19 @@ def test_xyz_round_trip(tmp_path):
     syms, xyz = read_xyz(p)
     assert syms == [m.symbol(i) for i in range(m.natom())]
     assert np.abs(xyz - m.geometry()).max() < 1e-14
+
+
+def test_fock_eigensolver_host_and_device_paths_agree():
+    """The SCF's Fock diagonalisation (LAPACK on the host for small n) gives
+    the eigenpairs torch.linalg.eigh gives."""
+    import torch
+    from quax_amd.methods.hartree_fock import _eigh
+    rng = np.random.default_rng(3)
+    X = rng.standard_normal((40, 40))
+    F = torch.tensor(X + X.T, dtype=torch.float64)
+    w1, v1 = _eigh(F, "host")
+    w2, v2 = _eigh(F, "device")
+    assert torch.allclose(w1, w2, atol=1e-12)
+    # eigenvectors up to sign: compare projectors onto the lowest 10
+    P1, P2 = v1[:, :10] @ v1[:, :10].T, v2[:, :10] @ v2[:, :10].T
+    assert torch.allclose(P1, P2, atol=1e-10)
