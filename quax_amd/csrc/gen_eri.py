@@ -221,12 +221,16 @@ def interleave(hlines, groups):
     return out
 
 
-def emit_dblocks(L, nn):
-    """Loads the six density blocks of the quartet (double) from A_.e.D."""
+def emit_dblocks(L, nn, arrays=True):
+    """Loads the six density blocks of the quartet (double) from A_.e.D into
+    local arrays; arrays=False only sets up the indices (the caller reads D
+    where each value is used)."""
     NA, NB, NC, ND = nn
     L.append("  const int n_ = A_.e.nbf;")
     L.append("  const double* Dm_ = A_.e.D;")
     L.append("  const int ia_ = A_.pr.bfa[P1], ib_ = A_.pr.bfb[P1], ic_ = A_.pr.bfa[P2], id_ = A_.pr.bfb[P2];")
+    if not arrays:
+        return
     for nm, r0, nr, c0, nc in (("Dab", "ia_", NA, "ib_", NB), ("Dcd", "ic_", NC, "id_", ND),
                                ("Dac", "ia_", NA, "ic_", NC), ("Dad", "ia_", NA, "id_", ND),
                                ("Dbc", "ib_", NB, "ic_", NC), ("Dbd", "ib_", NB, "id_", ND)):
@@ -263,6 +267,11 @@ def deriv_targets(l, part):
                             T.append(tuple(dn))
     return T
 
+
+# streaming J/K bodies of classes with at least this many integrals read each
+# density value where it is used instead of preloading the six blocks (the
+# (dd|dp) blocks alone are 162 doubles, i.e. 324 of the 512 registers)
+JK_LAZY_D_MIN = int(os.environ.get("QX_JK_LAZY_D_MIN", "324"))
 
 # largest SSA node count of one streaming J/K body; above it the class is
 # split over ranges of a components (as the derivative parts are).  Measured
@@ -414,7 +423,18 @@ def emit_eri_core(l):
         _head(J, f"qx_jk_e{nm}_{pi}", ", double (&jab)[%d], double (&jcd)[%d], double (&kac)[%d], double (&kad)[%d], "
               "double (&kbc)[%d], double (&kbd)[%d]" % (NA * NB, NC * ND, NA * NC, NA * ND, NB * NC, NB * ND),
               Lt, vp, vnp)
-        emit_dblocks(J, nn)
+        lazy = N >= JK_LAZY_D_MIN
+        emit_dblocks(J, nn, arrays=not lazy)
+
+        def dv(blk, r, c):
+            """density value: a preloaded block entry, or a load at the use"""
+            if not lazy:
+                nc_ = {"Dab": NB, "Dcd": ND, "Dac": NC, "Dad": ND, "Dbc": NC, "Dbd": ND}[blk]
+                return f"{blk}[{r * nc_ + c}]"
+            row, col = {"Dab": ("ia_", "ib_"), "Dcd": ("ic_", "id_"), "Dac": ("ia_", "ic_"),
+                        "Dad": ("ia_", "id_"), "Dbc": ("ib_", "ic_"), "Dbd": ("ib_", "id_")}[blk]
+            return f"Dm_[(size_t)({row} + {r}) * n_ + {col} + {c}]"
+
         groups = []
         k = 0
         for i in range(a0, a1):
@@ -425,9 +445,9 @@ def emit_eri_core(l):
                                                   j * ND + ll)
                         groups.append((posp[k], [
                             f"  {{ const double v_ = (double){outp[k]};",
-                            f"    jab[{ij}] += v_ * Dcd[{kl}]; jcd[{kl}] += v_ * Dab[{ij}];",
-                            f"    kac[{ik}] += v_ * Dbd[{jl}]; kad[{il}] += v_ * Dbc[{jk}];",
-                            f"    kbc[{jk}] += v_ * Dad[{il}]; kbd[{jl}] += v_ * Dac[{ik}]; }}"]))
+                            f"    jab[{ij}] += v_ * {dv('Dcd', kk, ll)}; jcd[{kl}] += v_ * {dv('Dab', i, j)};",
+                            f"    kac[{ik}] += v_ * {dv('Dbd', j, ll)}; kad[{il}] += v_ * {dv('Dbc', j, kk)};",
+                            f"    kbc[{jk}] += v_ * {dv('Dad', i, ll)}; kbd[{jl}] += v_ * {dv('Dac', i, kk)}; }}"]))
                         k += 1
         J += ["  " + x if not x.startswith("  ") else x for x in interleave(hp.lines, groups)]
         J.append("}")
