@@ -294,7 +294,10 @@ __global__ void k_oei(QxShellsDev sh, double* S, double* T, double* V) {
 // b; the nuclear-attraction operator centre follows from translational
 // invariance of each nucleus term.  With Pd/Wd/grad the contraction
 // grad += Pd.d(T+V) + Wd.dS is accumulated; with dS/dT/dV the full 3N x n x n
-// tensors are accumulated.
+// tensors are accumulated.  Thread x takes one ordered shell pair; grid row y
+// takes the terms: 0 the overlap and kinetic ones, 1 + C the attraction of
+// nucleus C (rows stride by gridDim.y), so a pair's nucleus loop is spread
+// over the grid instead of run by one lane.
 __global__ void k_oei_deriv(QxShellsDev sh, const double* Pd, const double* Wd, double* grad, double* dS, double* dT,
                             double* dV) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -305,52 +308,57 @@ __global__ void k_oei_deriv(QxShellsDev sh, const double* Pd, const double* Wd, 
   const int n = sh.ncart;
   const size_t nn = (size_t)n * n;
   const int atA = sh.atom[sa], atB = sh.atom[sb];
-  double gAB[2][3] = {{0, 0, 0}, {0, 0, 0}};
   double s1[3][OE_LMAX + 2][OE_JMAX + 2];
   double E[84];
-  for (int ia = sh.poff[sa]; ia < sh.poff[sa] + sh.nprim[sa]; ++ia)
-    for (int ib = sh.poff[sb]; ib < sh.poff[sb] + sh.nprim[sb]; ++ib) {
-      const OePrim P = oe_prim(sh, sa, sb, ia, ib);
-      oe_s1d(P, la + 1, lb + 3, s1);
-      for (int i = 0; i < na; ++i)
-        for (int j = 0; j < nb; ++j) {
-          int a[3], b[3];
-          unpack(sh.f_comp[sh.bfc[sa] + i], a);
-          unpack(sh.f_comp[sh.bfc[sb] + j], b);
-          const size_t rc = (size_t)(sh.bfc[sa] + i) * n + sh.bfc[sb] + j;
-          const double pd = Pd ? Pd[rc] : 0.0, wd = Wd ? Wd[rc] : 0.0;
-          for (int side = 0; side < 2; ++side) {
-            const double ex2 = 2.0 * (side == 0 ? P.a : P.b);
-            for (int x = 0; x < 3; ++x) {
-              int au[3] = {a[0], a[1], a[2]}, bu[3] = {b[0], b[1], b[2]};
-              if (side == 0) au[x] += 1; else bu[x] += 1;
-              double su, tu;
-              oe_st(P, au, bu, s1, su, tu);
-              double ds = ex2 * su, dt = ex2 * tu;
-              const int q = side == 0 ? a[x] : b[x];
-              if (q > 0) {
-                int ad[3] = {a[0], a[1], a[2]}, bd[3] = {b[0], b[1], b[2]};
-                if (side == 0) ad[x] -= 1; else bd[x] -= 1;
-                double sd, td;
-                oe_st(P, ad, bd, s1, sd, td);
-                ds -= q * sd;
-                dt -= q * td;
-              }
-              ds *= P.K;
-              dt *= P.K;
-              const int at = side == 0 ? atA : atB;
-              if (grad) gAB[side][x] += pd * dt + wd * ds;
-              if (dS) {
-                atomicAdd(&dS[(size_t)(3 * at + x) * nn + rc], ds);
-                atomicAdd(&dT[(size_t)(3 * at + x) * nn + rc], dt);
+  for (int term = blockIdx.y; term <= sh.natom; term += gridDim.y) {
+    double gAB[2][3] = {{0, 0, 0}, {0, 0, 0}};
+    for (int ia = sh.poff[sa]; ia < sh.poff[sa] + sh.nprim[sa]; ++ia)
+      for (int ib = sh.poff[sb]; ib < sh.poff[sb] + sh.nprim[sb]; ++ib) {
+        const OePrim P = oe_prim(sh, sa, sb, ia, ib);
+        if (term == 0) {  // overlap and kinetic
+          oe_s1d(P, la + 1, lb + 3, s1);
+          for (int i = 0; i < na; ++i)
+            for (int j = 0; j < nb; ++j) {
+              int a[3], b[3];
+              unpack(sh.f_comp[sh.bfc[sa] + i], a);
+              unpack(sh.f_comp[sh.bfc[sb] + j], b);
+              const size_t rc = (size_t)(sh.bfc[sa] + i) * n + sh.bfc[sb] + j;
+              const double pd = Pd ? Pd[rc] : 0.0, wd = Wd ? Wd[rc] : 0.0;
+              for (int side = 0; side < 2; ++side) {
+                const double ex2 = 2.0 * (side == 0 ? P.a : P.b);
+                for (int x = 0; x < 3; ++x) {
+                  int au[3] = {a[0], a[1], a[2]}, bu[3] = {b[0], b[1], b[2]};
+                  if (side == 0) au[x] += 1; else bu[x] += 1;
+                  double su, tu;
+                  oe_st(P, au, bu, s1, su, tu);
+                  double ds = ex2 * su, dt = ex2 * tu;
+                  const int q = side == 0 ? a[x] : b[x];
+                  if (q > 0) {
+                    int ad[3] = {a[0], a[1], a[2]}, bd[3] = {b[0], b[1], b[2]};
+                    if (side == 0) ad[x] -= 1; else bd[x] -= 1;
+                    double sd, td;
+                    oe_st(P, ad, bd, s1, sd, td);
+                    ds -= q * sd;
+                    dt -= q * td;
+                  }
+                  ds *= P.K;
+                  dt *= P.K;
+                  const int at = side == 0 ? atA : atB;
+                  if (grad) gAB[side][x] += pd * dt + wd * ds;
+                  if (dS) {
+                    atomicAdd(&dS[(size_t)(3 * at + x) * nn + rc], ds);
+                    atomicAdd(&dT[(size_t)(3 * at + x) * nn + rc], dt);
+                  }
+                }
               }
             }
-          }
+          continue;
         }
-      double Pc[3];
-      const double* A = sh.xyz + 3 * sh.atom[sa];
-      for (int d = 0; d < 3; ++d) Pc[d] = A[d] + P.PA[d];
-      for (int atC = 0; atC < sh.natom; ++atC) {
+        // attraction of nucleus atC
+        const int atC = term - 1;
+        double Pc[3];
+        const double* A = sh.xyz + 3 * sh.atom[sa];
+        for (int d = 0; d < 3; ++d) Pc[d] = A[d] + P.PA[d];
         oe_vrr(P, Pc, sh.xyz + 3 * atC, la + lb + 1, E);
         double gC[3] = {0, 0, 0};
         for (int i = 0; i < na; ++i)
@@ -389,12 +397,12 @@ __global__ void k_oei_deriv(QxShellsDev sh, const double* Pd, const double* Wd, 
           for (int x = 0; x < 3; ++x)
             if (gC[x] != 0.0) atomicAdd(&grad[3 * atC + x], gC[x]);
       }
-    }
-  if (grad)
-    for (int x = 0; x < 3; ++x) {
-      atomicAdd(&grad[3 * atA + x], gAB[0][x]);
-      atomicAdd(&grad[3 * atB + x], gAB[1][x]);
-    }
+    if (grad)
+      for (int x = 0; x < 3; ++x) {
+        atomicAdd(&grad[3 * atA + x], gAB[0][x]);
+        atomicAdd(&grad[3 * atB + x], gAB[1][x]);
+      }
+  }
 }
 
 inline unsigned nblk(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
@@ -441,8 +449,9 @@ int qxk_oei(const QxShellsDev* sh, double* S, double* T, double* V, hipStream_t 
 
 int qxk_oei_deriv(const QxShellsDev* sh, const double* Pd, const double* Wd, double* grad, double* dS, double* dT,
                   double* dV, hipStream_t s) {
-  hipLaunchKernelGGL(k_oei_deriv, dim3(nblk((long long)sh->nshell * sh->nshell, 64)), dim3(64), 0, s, *sh, Pd, Wd, grad,
-                     dS, dT, dV);
+  const unsigned rows = (unsigned)std::min(sh->natom + 1, 65535);
+  hipLaunchKernelGGL(k_oei_deriv, dim3(nblk((long long)sh->nshell * sh->nshell, 64), rows), dim3(64), 0, s, *sh, Pd, Wd,
+                     grad, dS, dT, dV);
   return (int)hipGetLastError();
 }
 
