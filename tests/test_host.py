@@ -193,3 +193,20 @@ def test_fock_eigensolver_host_and_device_paths_agree():
     # eigenvectors up to sign: compare projectors onto the lowest 10
     P1, P2 = v1[:, :10] @ v1[:, :10].T, v2[:, :10] @ v2[:, :10].T
     assert torch.allclose(P1, P2, atol=1e-10)
+
+
+def test_fock_eigensolver_occupied_subset():
+    """The occupied-only host solve (dsyevr) gives the lowest eigenvalues and
+    the same occupied projector (the density) as the full solve."""
+    import torch
+    from quax_amd.methods.hartree_fock import _eigh
+    rng = np.random.default_rng(5)
+    X = rng.standard_normal((60, 60))
+    F = torch.tensor(X + X.T, dtype=torch.float64)
+    w, v = _eigh(F, "host")
+    ws, vs = _eigh(F, "host", nocc=12)
+    assert ws.shape == (12,) and vs.shape == (60, 12)
+    assert torch.allclose(ws, w[:12], atol=1e-12)
+    assert torch.allclose(vs @ vs.T, v[:, :12] @ v[:, :12].T, atol=1e-12)
+    wf, vf = _eigh(F, "host", nocc=60)      # nocc = n falls back to the full solve
+    assert wf.shape == (60,)
