@@ -26,7 +26,7 @@ _PAD = 1.0e4  # diagonal of the padding rows: above every occupied orbital energ
 
 
 def batched_rhf(molecules, basis_name="even-tempered", pure=True, ints_tol=1e-12, conv=1e-10, maxit=200,
-                diis_space=8, precision="fp64", fp32_threshold=1e-8, gradient=True, device=0):
+                diis_space=20, precision="fp64", fp32_threshold=1e-8, gradient=True, device=0):
     """Returns (energies, gradients or None, aux) for a list of neutral
     closed-shell molecules; aux holds iterations per molecule."""
     import torch

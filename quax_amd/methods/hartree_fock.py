@@ -31,7 +31,7 @@ DEFAULT_OPTIONS = {
     "electric_field": 0,
     # additions of this implementation
     "diis": False,
-    "diis_space": 8,
+    "diis_space": 20,             # cluster10 at conv 1e-10: 53 iterations with 8 vectors, 41 with 20
     "precision": "fp64",          # or "mixed"
     "fp32_threshold": 1.0e-8,
     "conv": None,                 # override the reference's 1e-9 / 1e-10
