@@ -7,9 +7,12 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <exception>
 #include <functional>
 #include <memory>
+#include <new>
 #include <numeric>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -254,19 +257,28 @@ void run_launches(qx_ctx* c, std::vector<Launch>& v, hipStream_t s, const char* 
   for (int k = 0; k < ns - 1; ++k) QX_HIP(hipStreamWaitEvent(c->side[k], c->fork_ev[0], 0));
   std::vector<double> load(ns, 0.0);
   int rc = 0;
+  std::exception_ptr thrown;
   for (auto& x : v) {
     const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
     load[k] += x.cost;
-    rc = x.go(k == 0 ? s : c->side[k - 1]);
-    if (rc != 0) break;
+    try {
+      rc = x.go(k == 0 ? s : c->side[k - 1]);
+    } catch (...) {
+      thrown = std::current_exception();
+    }
+    if (rc != 0 || thrown) break;
   }
-  // join even after a failed launch, so the caller's stream never runs
-  // ahead of work already queued on the side streams
+  // join even after a failed or throwing launch, so the caller's stream
+  // never runs ahead of work already queued on the side streams
+  hipError_t join_err = hipSuccess;
   for (int k = 0; k < ns - 1; ++k) {
-    QX_HIP(hipEventRecord(c->fork_ev[k + 1], c->side[k]));
-    QX_HIP(hipStreamWaitEvent(s, c->fork_ev[k + 1], 0));
+    hipError_t e = hipEventRecord(c->fork_ev[k + 1], c->side[k]);
+    if (e == hipSuccess) e = hipStreamWaitEvent(s, c->fork_ev[k + 1], 0);
+    if (e != hipSuccess && join_err == hipSuccess) join_err = e;
   }
+  if (thrown) std::rethrow_exception(thrown);
   check_launch(rc, what);
+  QX_HIP(join_err);
 }
 
 void check_launch(int rc, const char* what) {
@@ -486,6 +498,12 @@ void schwarz_and_lists(qx_ctx* c) {
   }
   c->d_prefix = dupload(c, allpre);
   if (c->fragmented) c->d_kstart = dupload(c, allkst);
+  c->h_prefix = allpre;
+  c->h_kstart = allkst;
+  c->h_kstart_off = kst_off;
+  c->h_ppair.assign(c->npseudo, 0);
+  for (int i = 0; i < c->npairs; ++i)
+    for (int k = 0; k < c->h_pp_n[i]; ++k) c->h_ppair[c->h_pp_first[i] + k] = i;
   for (size_t k = 0; k < c->work.size(); ++k) {
     auto& wh = c->work[k];
     wh.w.prefix = c->d_prefix + wh.prefix_off;
@@ -623,17 +641,44 @@ void setup_shell_tables(qx_ctx* c) {
   s.sph = dupload(c, blk);
 }
 
-int fail(qx_ctx* c, const QxError& e) {
-  if (c) c->err = e.msg;
-  return e.code;
+thread_local std::string g_init_error;
+
+int fail(qx_ctx* c, int code, const char* msg) noexcept {
+  try {
+    if (c) c->err = msg; else g_init_error = msg;
+  } catch (...) {
+    // the message itself could not be stored; the code still reports the failure
+  }
+  return code;
 }
 
-thread_local std::string g_init_error;
+int fail(qx_ctx* c, const QxError& e) noexcept { return fail(c, e.code, e.msg.c_str()); }
+
+// Runs the body of a C-ABI entry point: every C++ exception becomes a status
+// code (nothing crosses the C ABI, so nothing reaches std::terminate).
+template <class F>
+int guarded(qx_ctx* c, F&& body) noexcept {
+  try {
+    return body();
+  } catch (const QxError& e) {
+    return fail(c, e);
+  } catch (const std::bad_alloc&) {
+    return fail(c, QX_ENOMEM, "host memory allocation failed");
+  } catch (const std::exception& e) {
+    return fail(c, QX_EHIP, e.what());
+  } catch (...) {
+    return fail(c, QX_EHIP, "unexpected C++ exception");
+  }
+}
 
 }  // namespace
 
 // ============================================================== C ABI
 extern "C" {
+
+static void init_body(qx_ctx* c, int natom, const double* xyz_bohr, const double* nuclear_charges, int nshell,
+                      const int* shell_l, const int* shell_atom, const int* shell_nprim, const double* exps,
+                      const double* coefs, int pure, double ints_tol, int device, const int* atom_fragment);
 
 static int init_impl(int natom, const double* xyz_bohr, const double* nuclear_charges, int nshell,
                      const int* shell_l, const int* shell_atom, const int* shell_nprim, const double* exps,
@@ -641,18 +686,39 @@ static int init_impl(int natom, const double* xyz_bohr, const double* nuclear_ch
                      qx_ctx** out) {
   if (!out) return QX_EINVAL;
   *out = nullptr;
-  qx_ctx* c = new qx_ctx();
-  try {
+  qx_ctx* c = nullptr;
+  const int rc = guarded(nullptr, [&]() -> int {
+    try {
+      c = new qx_ctx();
+      init_body(c, natom, xyz_bohr, nuclear_charges, nshell, shell_l, shell_atom, shell_nprim, exps, coefs, pure,
+                ints_tol, device, atom_fragment);
+    } catch (...) {
+      if (c) qx_finalize(c);
+      c = nullptr;
+      throw;
+    }
+    return QX_OK;
+  });
+  if (rc == QX_OK) *out = c;
+  return rc;
+}
+
+static void init_body(qx_ctx* c, int natom, const double* xyz_bohr, const double* nuclear_charges, int nshell,
+                      const int* shell_l, const int* shell_atom, const int* shell_nprim, const double* exps,
+                      const double* coefs, int pure, double ints_tol, int device, const int* atom_fragment) {
+  {
     QX_REQUIRE(natom > 0 && xyz_bohr && nuclear_charges, QX_EINVAL, "natom must be positive with coordinates");
     QX_REQUIRE(nshell > 0 && shell_l && shell_atom && shell_nprim && exps && coefs, QX_EINVAL, "no shells");
     QX_REQUIRE(ints_tol > 0 && std::isfinite(ints_tol), QX_EINVAL, "ints_tol must be positive");
+    // host-side tables first (a host allocation failure is reported before
+    // any device work starts)
+    c->natom = natom;
+    c->frag.assign(natom, 0);
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw QxError{QX_ENODEVICE, "no HIP device"};
     QX_REQUIRE(device >= 0 && device < ndev, QX_EINVAL, "device index out of range");
     QX_HIP(hipSetDevice(device));
     c->device = device;
-    c->natom = natom;
-    c->frag.assign(natom, 0);
     if (atom_fragment) {
       for (int a = 0; a < natom; ++a) {
         QX_REQUIRE(atom_fragment[a] >= 0, QX_EINVAL, "fragment ids must be non-negative");
@@ -694,14 +760,7 @@ static int init_impl(int natom, const double* xyz_bohr, const double* nuclear_ch
     setup_shell_tables(c);
     schwarz_and_lists(c);
     QX_HIP(hipDeviceSynchronize());
-  } catch (const QxError& e) {
-    g_init_error = e.msg;
-    int code = e.code;
-    qx_finalize(c);
-    return code;
   }
-  *out = c;
-  return QX_OK;
 }
 
 int qx_initialize(int natom, const double* xyz_bohr, const double* nuclear_charges, int nshell,
@@ -808,7 +867,7 @@ static void ao_tensor4_from_cart(qx_ctx* c, const double* cart, double* out, lon
 
 int qx_compute_2e_int(qx_ctx* c, double* out, void* stream) {
   if (!c || !out) return QX_EINVAL;
-  try {
+  return guarded(c, [&]() -> int {
     QX_HIP(hipSetDevice(c->device));
     hipStream_t s = (hipStream_t)stream;
     const size_t nc = c->ncart;
@@ -821,10 +880,8 @@ int qx_compute_2e_int(qx_ctx* c, double* out, void* stream) {
     e.epi = QX_EPI_FULL; e.atomic = c->k1 ? 0 : 1; e.nbf = (int)nc; e.natom = c->natom; e.coord = -1; e.out = dst;
     for (auto& wh : c->work) check_launch(launch_eri(c, wh.w, wh.cls, e, 0, s, c->pr, c->pp), "compute_2e_int");
     if (c->pure) ao_tensor4_from_cart(c, dst, out, 1, s);
-  } catch (const QxError& e) {
-    return fail(c, e);
-  }
-  return QX_OK;
+    return QX_OK;
+  });
 }
 
 // Derivative kernels (every part of every class) over one set of lists.
@@ -893,54 +950,47 @@ static void deriv_tensor(qx_ctx* c, int coord, double* out, hipStream_t s) {
 int qx_eri_deriv_core(qx_ctx* c, int order, double* out, void* stream) {
   if (!c || !out) return QX_EINVAL;
   if (order != 1) {
-    c->err = "eri_deriv_core: only deriv_order 1 is built";
-    return QX_EUNSUPPORTED;
+    return fail(c, QX_EUNSUPPORTED, "eri_deriv_core: only deriv_order 1 is built");
   }
-  try {
+  return guarded(c, [&]() -> int {
     QX_HIP(hipSetDevice(c->device));
     deriv_tensor(c, -1, out, (hipStream_t)stream);
-  } catch (const QxError& e) {
-    return fail(c, e);
-  }
-  return QX_OK;
+    return QX_OK;
+  });
 }
 
 int qx_compute_2e_deriv(qx_ctx* c, const int* dv, int len, double* out, void* stream) {
   if (!c || !out || !dv) return QX_EINVAL;
   if (len != 3 * c->natom) {
-    c->err = "compute_2e_deriv: deriv_vec length must be 3*natom";
-    return QX_EINVAL;
+    return fail(c, QX_EINVAL, "compute_2e_deriv: deriv_vec length must be 3*natom");
   }
   int order = 0, coord = -1;
   for (int i = 0; i < len; ++i) {
-    if (dv[i] < 0) { c->err = "compute_2e_deriv: negative deriv_vec entry"; return QX_EINVAL; }
+    if (dv[i] < 0) return fail(c, QX_EINVAL, "compute_2e_deriv: negative deriv_vec entry");
     order += dv[i];
     if (dv[i]) coord = i;
   }
   if (order > 1) {
-    c->err = "compute_2e_deriv: only first-order deriv_vec is built";
-    return QX_EUNSUPPORTED;
+    return fail(c, QX_EUNSUPPORTED, "compute_2e_deriv: only first-order deriv_vec is built");
   }
-  try {
+  return guarded(c, [&]() -> int {
     QX_HIP(hipSetDevice(c->device));
     hipStream_t s = (hipStream_t)stream;
     if (order == 0) {
       // the reference's order-0 call returns buffer 0 of a deriv-0 engine, i.e.
       // the integrals, with single-atom quartets skipped (libint_interface.cc:1180-1185)
-      return QX_EUNSUPPORTED;
+      return fail(c, QX_EUNSUPPORTED, "compute_2e_deriv: an all-zero deriv_vec is not built");
     }
     deriv_tensor(c, coord, out, s);
-  } catch (const QxError& e) {
-    return fail(c, e);
-  }
-  return QX_OK;
+    return QX_OK;
+  });
 }
 
 int qx_jk_build(qx_ctx* c, const double* D, double* J, double* K, int precision, double thr, void* stream) {
   if (!c || !D || !J || !K) return QX_EINVAL;
   if (precision != QX_PREC_FP64 && precision != QX_PREC_MIXED) return QX_EINVAL;
   if (precision == QX_PREC_MIXED && !(thr >= 0.0 && std::isfinite(thr))) return QX_EINVAL;
-  try {
+  return guarded(c, [&]() -> int {
     QX_HIP(hipSetDevice(c->device));
     hipStream_t s = (hipStream_t)stream;
     const size_t nc = c->ncart, n2 = nc * nc;
@@ -975,17 +1025,15 @@ int qx_jk_build(qx_ctx* c, const double* D, double* J, double* K, int precision,
       check_launch(qxk_sym_half(Kp, K, (int)nc, s), "sym");
     }
     end_ws(c, s);
-  } catch (const QxError& e) {
-    return fail(c, e);
-  }
-  return QX_OK;
+    return QX_OK;
+  });
 }
 
 int qx_eri_grad(qx_ctx* c, const double* D, double* grad, int precision, double thr, void* stream) {
   if (!c || !D || !grad) return QX_EINVAL;
   if (precision != QX_PREC_FP64 && precision != QX_PREC_MIXED) return QX_EINVAL;
   if (precision == QX_PREC_MIXED && !(thr >= 0.0 && std::isfinite(thr))) return QX_EINVAL;
-  try {
+  return guarded(c, [&]() -> int {
     QX_HIP(hipSetDevice(c->device));
     hipStream_t s = (hipStream_t)stream;
     const size_t nc = c->ncart, n2 = nc * nc;
@@ -1007,15 +1055,13 @@ int qx_eri_grad(qx_ctx* c, const double* D, double* grad, int precision, double 
     }
     run_launches(c, v, s, "eri_grad");
     end_ws(c, s);
-  } catch (const QxError& e) {
-    return fail(c, e);
-  }
-  return QX_OK;
+    return QX_OK;
+  });
 }
 
 int qx_oei(qx_ctx* c, double* S, double* T, double* V, void* stream) {
   if (!c || !S || !T || !V) return QX_EINVAL;
-  try {
+  return guarded(c, [&]() -> int {
     QX_HIP(hipSetDevice(c->device));
     hipStream_t s = (hipStream_t)stream;
     const size_t nc = c->ncart;
@@ -1029,15 +1075,13 @@ int qx_oei(qx_ctx* c, double* S, double* T, double* V, void* stream) {
       check_launch(qxk_cart_to_ao_mat(&c->shells, d.d(), V, s), "cart_to_ao");
       QX_HIP(hipStreamSynchronize(s));
     }
-  } catch (const QxError& e) {
-    return fail(c, e);
-  }
-  return QX_OK;
+    return QX_OK;
+  });
 }
 
 int qx_oei_grad(qx_ctx* c, const double* Pd, const double* Wd, double* grad, void* stream) {
   if (!c || !Pd || !Wd || !grad) return QX_EINVAL;
-  try {
+  return guarded(c, [&]() -> int {
     QX_HIP(hipSetDevice(c->device));
     hipStream_t s = (hipStream_t)stream;
     const size_t nc = c->ncart;
@@ -1050,19 +1094,16 @@ int qx_oei_grad(qx_ctx* c, const double* Pd, const double* Wd, double* grad, voi
       check_launch(qxk_oei_deriv(&c->shells, a.d(), b.d(), grad, nullptr, nullptr, nullptr, s), "oei_grad");
       QX_HIP(hipStreamSynchronize(s));
     }
-  } catch (const QxError& e) {
-    return fail(c, e);
-  }
-  return QX_OK;
+    return QX_OK;
+  });
 }
 
 int qx_oei_deriv(qx_ctx* c, double* dS, double* dT, double* dV, void* stream) {
   if (!c || !dS || !dT || !dV) return QX_EINVAL;
   if (c->pure) {
-    c->err = "oei_deriv: Cartesian bases only";
-    return QX_EUNSUPPORTED;
+    return fail(c, QX_EUNSUPPORTED, "oei_deriv: Cartesian bases only");
   }
-  try {
+  return guarded(c, [&]() -> int {
     QX_HIP(hipSetDevice(c->device));
     hipStream_t s = (hipStream_t)stream;
     const long long n = 3LL * c->natom * c->ncart * c->ncart;
@@ -1070,10 +1111,8 @@ int qx_oei_deriv(qx_ctx* c, double* dS, double* dT, double* dV, void* stream) {
     check_launch(qxk_fill(dT, n, 0.0, s), "fill");
     check_launch(qxk_fill(dV, n, 0.0, s), "fill");
     check_launch(qxk_oei_deriv(&c->shells, nullptr, nullptr, nullptr, dS, dT, dV, s), "oei_deriv");
-  } catch (const QxError& e) {
-    return fail(c, e);
-  }
-  return QX_OK;
+    return QX_OK;
+  });
 }
 
 int qx_store_size(const qx_ctx* c, long long* nd) {
@@ -1085,17 +1124,55 @@ int qx_store_size(const qx_ctx* c, long long* nd) {
 int qx_eri_store(qx_ctx* c, double* out, void* stream) {
   if (!c || !out) return QX_EINVAL;
   if (!c->k1) {
-    c->err = "eri_store: single-primitive shells only";
-    return QX_EUNSUPPORTED;
+    return fail(c, QX_EUNSUPPORTED, "eri_store: single-primitive shells only");
   }
-  try {
+  return guarded(c, [&]() -> int {
     QX_HIP(hipSetDevice(c->device));
     hipStream_t s = (hipStream_t)stream;
     QxEpiDev e{};
     e.epi = QX_EPI_STORE; e.nbf = c->ncart; e.natom = c->natom; e.coord = -1; e.out = out;
     for (auto& wh : c->work) check_launch(launch_eri(c, wh.w, wh.cls, e, 0, s, c->pr, c->pp), "eri_store");
-  } catch (const QxError& e) {
-    return fail(c, e);
+    return QX_OK;
+  });
+}
+
+int qx_store_lists(const qx_ctx* c, int* nlists, int* l4, long long* base, long long* nquart) {
+  if (!c || !nlists) return QX_EINVAL;
+  *nlists = (int)c->work.size();
+  for (size_t k = 0; k < c->work.size(); ++k) {
+    const QxWorkHost& wh = c->work[k];
+    const QxClassEntry& e = qx_class_table[wh.cls];
+    if (l4) {
+      l4[4 * k] = e.la; l4[4 * k + 1] = e.lb; l4[4 * k + 2] = e.lc; l4[4 * k + 3] = e.ld;
+    }
+    if (base) base[k] = wh.w.store_base;
+    if (nquart) nquart[k] = wh.w.nquart;
+  }
+  return QX_OK;
+}
+
+int qx_store_quartet_pairs(const qx_ctx* c, int list, int* p1, int* p2) {
+  if (!c || !p1 || !p2 || list < 0 || list >= (int)c->work.size()) return QX_EINVAL;
+  const QxWorkHost& wh = c->work[list];
+  const long long* pre = c->h_prefix.data() + wh.prefix_off;
+  const int* kst = c->h_kstart.data() + c->h_kstart_off[list];
+  for (int lo = 0; lo < wh.w.nbra; ++lo) {
+    const int u1 = wh.w.bra_off + lo;
+    for (long long t = pre[lo]; t < pre[lo + 1]; ++t) {
+      const int u2 = wh.w.ket_off + kst[lo] + (int)(t - pre[lo]);
+      p1[t] = c->h_ppair[u1];
+      p2[t] = c->h_ppair[u2];
+    }
+  }
+  return QX_OK;
+}
+
+int qx_pair_shells(const qx_ctx* c, int* npairs, int* sa, int* sb) {
+  if (!c || !npairs) return QX_EINVAL;
+  *npairs = c->npairs;
+  for (int i = 0; i < c->npairs; ++i) {
+    if (sa) sa[i] = c->h_pair_sa[i];
+    if (sb) sb[i] = c->h_pair_sb[i];
   }
   return QX_OK;
 }
@@ -1129,15 +1206,13 @@ int qx_profile_slot(int slot, int* kind, int* cls, int* part, int* fp32) {
 
 int qx_mixed_quartets(qx_ctx* c, double thr, long long* n64, long long* n32) {
   if (!c || !(thr >= 0.0 && std::isfinite(thr))) return QX_EINVAL;
-  try {
+  return guarded(c, [&]() -> int {
     QX_HIP(hipSetDevice(c->device));
     build_split(c, thr);
     if (n64) *n64 = c->split.nq64;
     if (n32) *n32 = c->split.nq32;
-  } catch (const QxError& e) {
-    return fail(c, e);
-  }
-  return QX_OK;
+    return QX_OK;
+  });
 }
 
 int qx_set_streams(qx_ctx* c, int n) {
@@ -1198,23 +1273,22 @@ int qx_class_quartets(const qx_ctx* c, long long* eri, long long* deri) {
 
 int qx_profile_enable(qx_ctx* c, int on) {
   if (!c) return QX_EINVAL;
-  try {
-    double tmp[QX_PROFILE_SLOTS];
-    long long t1[QX_PROFILE_SLOTS], t2[QX_PROFILE_SLOTS];
-    if (!c->prof_pending.empty()) qx_profile_read(c, tmp, t1, t2);
+  return guarded(c, [&]() -> int {
+    if (!c->prof_pending.empty()) {
+      const int rc = qx_profile_read(c, nullptr, nullptr, nullptr);
+      if (rc != QX_OK) return rc;
+    }
     c->profiling = on != 0;
     c->prof_ms.assign(QX_PROFILE_SLOTS, 0.0);
     c->prof_launches.assign(QX_PROFILE_SLOTS, 0);
     c->prof_pq.assign(QX_PROFILE_SLOTS, 0);
-  } catch (const QxError& e) {
-    return fail(c, e);
-  }
-  return QX_OK;
+    return QX_OK;
+  });
 }
 
 int qx_profile_read(qx_ctx* c, double* ms, long long* launches, long long* pq) {
   if (!c) return QX_EINVAL;
-  try {
+  return guarded(c, [&]() -> int {
     if (c->prof_ms.size() != QX_PROFILE_SLOTS) {
       c->prof_ms.assign(QX_PROFILE_SLOTS, 0.0);
       c->prof_launches.assign(QX_PROFILE_SLOTS, 0);
@@ -1234,10 +1308,8 @@ int qx_profile_read(qx_ctx* c, double* ms, long long* launches, long long* pq) {
       if (launches) launches[k] = c->prof_launches[k];
       if (pq) pq[k] = c->prof_pq[k];
     }
-  } catch (const QxError& e) {
-    return fail(c, e);
-  }
-  return QX_OK;
+    return QX_OK;
+  });
 }
 
 }  // extern "C"

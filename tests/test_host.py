@@ -74,6 +74,42 @@ def test_library_rejects_without_gpu():
         EriEngine(mol, BasisSet.build(mol, "sto-3g"))
 
 
+_ENOMEM_PROBE = r"""
+import ctypes, resource, sys
+import numpy as np
+sys.path.insert(0, sys.argv[1])
+from quax_amd import _lib
+L = _lib.load()
+vm = [int(l.split()[1]) * 1024 for l in open('/proc/self/status') if l.startswith('VmSize:')][0]
+lim = vm + (1 << 30)
+resource.setrlimit(resource.RLIMIT_AS, (lim, lim))
+assert resource.getrlimit(resource.RLIMIT_AS)[0] == lim
+dp, ip = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
+xyz, Z, ex, co = np.zeros(3), np.ones(1), np.ones(1), np.ones(1)
+li = np.zeros(1, dtype=np.int32); at = np.zeros(1, dtype=np.int32); npr = np.ones(1, dtype=np.int32)
+ctx = ctypes.c_void_p()
+# 2**31 - 1 atoms: the per-atom host table alone needs 8.6 GB, far above the limit
+rc = L.qx_initialize(2**31 - 1, xyz.ctypes.data_as(dp), Z.ctypes.data_as(dp), 1, li.ctypes.data_as(ip),
+                     at.ctypes.data_as(ip), npr.ctypes.data_as(ip), ex.ctypes.data_as(dp), co.ctypes.data_as(dp),
+                     0, 1e-12, 0, ctypes.byref(ctx))
+print(rc, ctx.value, L.qx_last_error(None).decode())
+"""
+
+
+def test_host_allocation_failure_is_a_status_not_an_abort():
+    """A C++ exception inside an entry point comes back as a status code
+    (QX_ENOMEM for a failed host allocation); nothing reaches std::terminate."""
+    import subprocess
+    import sys
+    from quax_amd import _lib
+    _lib.load()
+    r = subprocess.run([sys.executable, "-c", _ENOMEM_PROBE, ROOT], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    rc, ptr, msg = r.stdout.strip().split(" ", 2)
+    assert int(rc) == _lib.QX_ENOMEM and ptr == "None"
+    assert "allocation" in msg
+
+
 def test_class_table_static_info():
     from quax_amd import _lib
     L = _lib.load()
