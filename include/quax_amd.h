@@ -144,6 +144,36 @@ int qx_oei_deriv(qx_ctx* ctx, double* dS, double* dT, double* dV, void* stream);
 int qx_store_size(const qx_ctx* ctx, long long* ndoubles);
 int qx_eri_store(qx_ctx* ctx, double* out, void* stream);
 
+/* Map of the stored layout (no reference counterpart; lets a caller read
+ * the class-blocked integrals of qx_eri_store), host outputs.
+ * qx_store_lists: number of quartet lists and, per list, its class
+ * (l4[4*k..4*k+3] = la, lb, lc, ld), element offset (base) and quartet
+ * count; arrays need qx_num_classes() entries.
+ * qx_store_quartet_pairs: for every quartet t of list `list`, in stored
+ * order, the bra and ket shell pair (indices into qx_pair_shells).
+ * qx_pair_shells: the screened shell pairs, oriented (l(sa) >= l(sb)); the
+ * stored components of a quartet run over (sa, sb | sa', sb') row-major. */
+int qx_store_lists(const qx_ctx* ctx, int* nlists, int* l4, long long* base, long long* nquart);
+int qx_store_quartet_pairs(const qx_ctx* ctx, int list, int* p1, int* p2);
+int qx_pair_shells(const qx_ctx* ctx, int* npairs, int* sa, int* sb);
+
+/* AO -> MO transformation of a two-electron tensor (replaces
+ * partial_tei_transformation and tei_transformation, quax/methods/
+ * energy_utils.py:46-73, called from mp2.py:29 and ccsd.py:32):
+ *   out[I,J,K,L] = sum_pqrs Ci[p,I] Cj[q,J] Ck[r,K] Cl[s,L] G[p,q,r,s].
+ * n[4] = dimensions of G (row-major, device), m[4] = MO counts; Ci is
+ * n[0] x m[0] row-major (device), likewise Cj, Ck, Cl; out holds
+ * m[0]*m[1]*m[2]*m[3] doubles (device).  Four FP64 MFMA GEMMs in the
+ * reference's order (l, k, j, i), each contracting the last index and
+ * putting the new one first.  work: device buffer of
+ * qx_tei_transform_work(n, m) doubles.  No context; errors in
+ * qx_gemm_last_error(). */
+long long qx_tei_transform_work(const int* n, const int* m);
+int qx_partial_tei_transformation(const double* G, const int* n, const double* Ci, const double* Cj,
+                                  const double* Ck, const double* Cl, const int* m, double* out, double* work,
+                                  void* stream);
+const char* qx_gemm_last_error(void);
+
 /* Mixed-precision split of the canonical screened quartets at
  * fp32_threshold: contracted quartets evaluated in FP64 and in FP32. */
 int qx_mixed_quartets(qx_ctx* ctx, double fp32_threshold, long long* n_fp64, long long* n_fp32);

@@ -57,6 +57,12 @@ SIGNATURES = {
     "qx_oei_deriv": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "qx_store_size": (_i, [_vp, ctypes.POINTER(_ll)]),
     "qx_eri_store": (_i, [_vp, _vp, _vp]),
+    "qx_store_lists": (_i, [_vp, _ip, _ip, ctypes.POINTER(_ll), ctypes.POINTER(_ll)]),
+    "qx_store_quartet_pairs": (_i, [_vp, _i, _ip, _ip]),
+    "qx_pair_shells": (_i, [_vp, _ip, _ip, _ip]),
+    "qx_tei_transform_work": (_ll, [_ip, _ip]),
+    "qx_partial_tei_transformation": (_i, [_vp, _ip, _vp, _vp, _vp, _vp, _ip, _vp, _vp, _vp]),
+    "qx_gemm_last_error": (ctypes.c_char_p, []),
     "qx_num_classes": (_i, []),
     "qx_profile_slots": (_i, []),
     "qx_profile_slot": (_i, [_i, _ip, _ip, _ip, _ip]),

@@ -71,6 +71,12 @@ struct qx_ctx {
   std::vector<QxWorkHost> work;
   long long* d_prefix = nullptr;
   int* d_kstart = nullptr;               // per bra pseudo pair, fragmented contexts only
+  // host copies of the list arrays (quartet -> pair maps for callers that
+  // read the stored integrals) and of the pseudo pair -> pair map
+  std::vector<long long> h_prefix;
+  std::vector<int> h_kstart;             // per bra pseudo pair of every list
+  std::vector<long long> h_kstart_off;   // per list: offset into h_kstart
+  std::vector<int> h_ppair;
   long long nquart_total = 0;            // canonical contracted quartets
   long long npq_total = 0;               // pseudo-pair quartets
   long long store_total = 0;

@@ -175,6 +175,34 @@ class EriEngine:
         check(self.L.qx_eri_store(self.ctx, ptr(out), self._stream()), self.ctx, "eri_store")
         return out
 
+    def store_map(self):
+        """Layout of ``store()``: a list of dicts, one per quartet list, with
+        the class ``l4``, element offset ``base``, quartet count ``nquart``
+        and, per quartet in stored order, the bra and ket shell pairs
+        (``p1``, ``p2``: indices into ``pairs``); plus ``pairs``, the
+        oriented (sa, sb) shells of every screened pair."""
+        nc = self.L.qx_num_classes()
+        nl = ctypes.c_int()
+        l4 = (ctypes.c_int * (4 * nc))()
+        base = (ctypes.c_longlong * nc)()
+        nq = (ctypes.c_longlong * nc)()
+        check(self.L.qx_store_lists(self.ctx, ctypes.byref(nl), l4, base, nq), self.ctx, "store_lists")
+        npair = ctypes.c_int()
+        check(self.L.qx_pair_shells(self.ctx, ctypes.byref(npair), None, None), self.ctx, "pair_shells")
+        sa = np.zeros(npair.value, dtype=np.int32)
+        sb = np.zeros(npair.value, dtype=np.int32)
+        ip = ctypes.POINTER(ctypes.c_int)
+        check(self.L.qx_pair_shells(self.ctx, ctypes.byref(npair), sa.ctypes.data_as(ip), sb.ctypes.data_as(ip)),
+              self.ctx, "pair_shells")
+        lists = []
+        for k in range(nl.value):
+            p1 = np.zeros(nq[k], dtype=np.int32)
+            p2 = np.zeros(nq[k], dtype=np.int32)
+            check(self.L.qx_store_quartet_pairs(self.ctx, k, p1.ctypes.data_as(ip), p2.ctypes.data_as(ip)),
+                  self.ctx, "store_quartet_pairs")
+            lists.append(dict(l4=tuple(l4[4 * k:4 * k + 4]), base=base[k], nquart=nq[k], p1=p1, p2=p2))
+        return lists, np.stack([sa, sb], axis=1)
+
     # ------------------------------------------------------------ direct
     def jk(self, D, precision: int = QX_PREC_FP64, fp32_threshold: float = 0.0, out=None):
         D = self._dev(D)

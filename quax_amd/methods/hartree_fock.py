@@ -102,10 +102,11 @@ def nuclear_repulsion_grad(geom, Z):
     return out.reshape(-1)
 
 
-def restricted_hartree_fock(engine, nelectrons, options=None, return_aux_data=False, jk=None):
+def restricted_hartree_fock(engine, nelectrons, options=None, return_aux_data=False, jk=None, D0=None):
     """Returns E_scf, or (E_scf, aux) with aux holding D, C, eps, F, S, H,
     iteration count.  ``jk(D) -> (J, K)`` replaces the engine's direct J/K
-    (used by tests to compare contraction paths)."""
+    (used by tests to compare contraction paths); ``D0`` is a starting
+    density (nbf x nbf) in place of the zero / core guess."""
     import torch
     opts = check_options(options)
     dev = engine.device
@@ -163,7 +164,12 @@ def restricted_hartree_fock(engine, nelectrons, options=None, return_aux_data=Fa
             return F
         return torch.tensordot(c[:m], torch.stack(diis_F), dims=1)
 
-    D = H.clone() if opts["guess_core"] else torch.zeros_like(H)
+    if D0 is not None:
+        D = torch.as_tensor(D0, dtype=dt).to(dev).clone()
+        if tuple(D.shape) != tuple(H.shape):
+            raise ValueError("D0 must be nbf x nbf")
+    else:
+        D = H.clone() if opts["guess_core"] else torch.zeros_like(H)
     J, K = build_jk(D)
     F = H + 2 * J - K
     D_old = D.clone()
