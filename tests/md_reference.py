@@ -140,6 +140,98 @@ def eri_primitive(la, a, A, lb, b, B, lc, c, C, ld, d, D):
     return out
 
 
+def _hermite_R_array(L, alpha, PQ):
+    """R_{tuv}^0 for t + u + v <= L as an (L+1)^3 array (zero outside)."""
+    R = hermite_R(L, L, L, alpha, PQ)
+    out = np.zeros((L + 1, L + 1, L + 1))
+    for t in range(L + 1):
+        for u in range(L + 1 - t):
+            for v in range(L + 1 - t - u):
+                out[t, u, v] = R(t, u, v, 0)
+    return out
+
+
+def eri_block_md(la, a, A, lb, b, B, lc, c, C, ld, d, D):
+    """Same primitive block as eri_primitive, vectorised over the Hermite
+    sums: (ab|cd) = pref * sum E^ab_tuv (-1)^(tau+nu+phi) E^cd_{tau nu phi}
+    R_{t+tau, u+nu, v+phi}."""
+    A, B, C, D = (np.asarray(x, float) for x in (A, B, C, D))
+    p, q = a + b, c + d
+    P = (a * A + b * B) / p
+    Q = (c * C + d * D) / q
+    alpha = p * q / (p + q)
+    L = la + lb + lc + ld
+    R = _hermite_R_array(L, alpha, P - Q)
+    n1, n2 = la + lb + 1, lc + ld + 1
+    t1, t2 = np.arange(n1), np.arange(n2)
+    idx = t1[:, None] + t2[None, :]
+    # H[t, u, v, tau, nu, phi] = R[t+tau, u+nu, v+phi]
+    H = R[idx[:, None, None, :, None, None], idx[None, :, None, None, :, None], idx[None, None, :, None, None, :]]
+
+    def evec(l1, l2, e1, e2, X1, X2, comps1, comps2, n, sign):
+        Ek = [{(i, j): np.pad(np.array(hermite_E(i, j, e1, e2, X1[k] - X2[k])), (0, n - i - j - 1))
+               for i in range(l1 + 1) for j in range(l2 + 1)} for k in range(3)]
+        out = np.zeros((len(comps1), len(comps2), n, n, n))
+        s = (-1.0) ** np.arange(n) if sign else np.ones(n)
+        for i1, c1 in enumerate(comps1):
+            for i2, c2 in enumerate(comps2):
+                ex, ey, ez = (Ek[k][(c1[k], c2[k])] * s for k in range(3))
+                out[i1, i2] = ex[:, None, None] * ey[None, :, None] * ez[None, None, :]
+        return out
+
+    Eab = evec(la, lb, a, b, A, B, cart_components(la), cart_components(lb), n1, False)
+    Ecd = evec(lc, ld, c, d, C, D, cart_components(lc), cart_components(ld), n2, True)
+    pref = 2 * math.pi ** 2.5 / (p * q * math.sqrt(p + q))
+    return pref * np.einsum("abtuv,tuvxyz,cdxyz->abcd", Eab, H, Ecd, optimize=True)
+
+
+def _shift_index(l):
+    """Cartesian component tuple -> index within shell l."""
+    return {c: i for i, c in enumerate(cart_components(l))}
+
+
+def eri_deriv1_contracted(shells):
+    """First derivatives d/dX_k of the Libint-normalised block w.r.t. the four
+    centres: array (4, 3, na, nb, nc, nd), from the Gaussian identity
+    d/dA_k x^a exp(-alpha |r-A|^2) = 2 alpha x^(a+1_k) ... - a_k x^(a-1_k) ...
+    applied to primitive MD blocks of the raised and lowered shells."""
+    norms = [normalize(l, e, c) for (l, e, c, _) in shells]
+    ls = [s[0] for s in shells]
+    nfun = [len(cart_components(l)) for l in ls]
+    out = np.zeros((4, 3) + tuple(nfun))
+    exps = [s[1] for s in shells]
+    cen = [np.asarray(s[3], float) for s in shells]
+    for i0, e0 in enumerate(exps[0]):
+        for i1, e1 in enumerate(exps[1]):
+            for i2, e2 in enumerate(exps[2]):
+                for i3, e3 in enumerate(exps[3]):
+                    w = norms[0][i0] * norms[1][i1] * norms[2][i2] * norms[3][i3]
+                    ee = (e0, e1, e2, e3)
+                    for k in range(4):
+                        def blk(lk):
+                            lv = list(ls)
+                            lv[k] = lk
+                            return eri_block_md(lv[0], ee[0], cen[0], lv[1], ee[1], cen[1], lv[2], ee[2], cen[2],
+                                                lv[3], ee[3], cen[3])
+                        up = blk(ls[k] + 1)
+                        dn = blk(ls[k] - 1) if ls[k] > 0 else None
+                        iu = _shift_index(ls[k] + 1)
+                        idn = _shift_index(ls[k] - 1) if ls[k] > 0 else None
+                        for f, comp in enumerate(cart_components(ls[k])):
+                            for x in range(3):
+                                cu = list(comp)
+                                cu[x] += 1
+                                val = 2 * ee[k] * np.take(up, iu[tuple(cu)], axis=k)
+                                if comp[x] > 0:
+                                    cd = list(comp)
+                                    cd[x] -= 1
+                                    val = val - comp[x] * np.take(dn, idn[tuple(cd)], axis=k)
+                                sl = [slice(None)] * 4
+                                sl[k] = f
+                                out[(k, x) + tuple(sl)] += w * val
+    return out
+
+
 def eri_contracted(shells):
     """shells: 4 tuples (l, exps, raw_coefs, centre).  Libint-normalised block."""
     blocks = None

@@ -91,6 +91,58 @@ def test_hgp_matches_mcmurchie_davidson(seed):
         assert np.abs(blk - ref).max() < 1e-14 * max(1.0, np.abs(ref).max())
 
 
+_CLASSES = [(la, lb, lc, ld) for la in range(3) for lb in range(la + 1) for lc in range(3) for ld in range(lc + 1)
+            if la * (la + 1) // 2 + lb >= lc * (lc + 1) // 2 + ld]
+
+
+def _class_quartet(cls, seed):
+    """Four shells of angular momenta cls on four random centres; contracted
+    (two primitives) where the MD reference stays fast, else one."""
+    rng = np.random.default_rng(seed)
+    mol = Molecule(["C", "H", "O", "N"], rng.uniform(-1.2, 1.2, (4, 3)))
+    k = 2 if sum(cls) <= 4 else 1
+    shells = [Shell(l, a, list(rng.uniform(0.3, 2.5, k)), list(rng.uniform(0.3, 1.0, k))) for a, l in enumerate(cls)]
+    return mol, shells
+
+
+@pytest.mark.parametrize("cls", _CLASSES, ids=lambda c: "".join(map(str, c)))
+def test_every_eri_class_matches_mcmurchie_davidson(cls):
+    """All 21 canonical classes with l <= 2: the oracle's HGP block against
+    the independent McMurchie-Davidson block (Hermite expansion, mpmath
+    Boys function)."""
+    mol, shells = _class_quartet(cls, 100 + sum(10 ** i * l for i, l in enumerate(cls)))
+    ob = oracle.OracleBasis(mol, BasisSet("t", shells))
+    blk = ob.shell_quartet(0, 1, 2, 3)
+    sh = [(s.l, s.exps, s.coefs, mol.geometry()[s.atom]) for s in shells]
+    ref = md.eri_contracted(sh) if sum(cls) <= 3 else None
+    norms = [md.normalize(s.l, s.exps, s.coefs) for s in shells]
+    vec = None
+    for i, a in enumerate(shells[0].exps):
+        for j, b in enumerate(shells[1].exps):
+            for kk, c in enumerate(shells[2].exps):
+                for m, d in enumerate(shells[3].exps):
+                    w = norms[0][i] * norms[1][j] * norms[2][kk] * norms[3][m]
+                    x = w * md.eri_block_md(cls[0], a, sh[0][3], cls[1], b, sh[1][3], cls[2], c, sh[2][3],
+                                            cls[3], d, sh[3][3])
+                    vec = x if vec is None else vec + x
+    if ref is not None:     # the two MD codes agree with each other
+        assert np.abs(vec - ref).max() < 1e-14 * max(1.0, np.abs(ref).max())
+    assert np.abs(blk - vec).max() < 1e-14 * max(1.0, np.abs(vec).max())
+
+
+@pytest.mark.parametrize("cls", _CLASSES, ids=lambda c: "".join(map(str, c)))
+def test_every_derivative_class_matches_mcmurchie_davidson(cls):
+    """First derivatives w.r.t. all four centres, every class with l <= 2:
+    the oracle against the Gaussian shift identity applied to MD blocks of
+    the raised and lowered shells."""
+    mol, shells = _class_quartet(cls, 200 + sum(10 ** i * l for i, l in enumerate(cls)))
+    ob = oracle.OracleBasis(mol, BasisSet("t", shells))
+    d = ob.shell_quartet_deriv1(0, 1, 2, 3)
+    ref = md.eri_deriv1_contracted([(s.l, s.exps, s.coefs, mol.geometry()[s.atom]) for s in shells])
+    ref = ref.reshape(d.shape)
+    assert np.abs(d - ref).max() < 1e-13 * max(1.0, np.abs(ref).max())
+
+
 def test_h2_sto3g_textbook():
     """Szabo & Ostlund, Modern Quantum Chemistry, ch. 3 (R = 1.4 bohr)."""
     mol = Molecule(["H", "H"], [[0, 0, 0], [0, 0, 1.4]])
