@@ -85,6 +85,57 @@ def kernel_table(eng):
     return info
 
 
+def flop_model():
+    """Algorithmic FLOPs per canonical quartet and class (textbook HGP
+    counts of the whole class, quax_amd/csrc/flop_model.json)."""
+    with open(os.path.join(ROOT, "quax_amd", "csrc", "flop_model.json")) as fh:
+        return json.load(fh)["classes"]
+
+
+def slot_algorithmic_flops(table, model, eng, slot):
+    """Algorithmic FLOPs per primitive quartet that one launch in `slot`
+    accounts for: the class's J/K count for a J/K kernel; for one part of a
+    split derivative class, the class's adjoint count shared out over its
+    parts in proportion to their executed counts."""
+    import ctypes
+    name, fl, nm, key = table[slot]
+    if key.startswith("jk"):
+        njk = ctypes.c_int()
+        jf = (ctypes.c_double * 8)()
+        cls = [k for k in range(eng.L.qx_num_classes()) if table_class_name(eng, k) == nm][0]
+        eng.L.qx_class_jk(cls, ctypes.byref(njk), jf)
+        return model[nm]["jk"] * fl / sum(jf[:njk.value])
+    if key.startswith("deriv"):
+        cls = [k for k in range(eng.L.qx_num_classes()) if table_class_name(eng, k) == nm][0]
+        ngr = ctypes.c_int()
+        gf = (ctypes.c_double * eng.L.qx_max_parts())()
+        gp = (ctypes.c_int * eng.L.qx_max_parts())()
+        eng.L.qx_class_groups(cls, ctypes.byref(ngr), gf, gp)
+        return model[nm]["adjoint"] * fl / sum(gf[:ngr.value])
+    return model[nm]["eri"]
+
+
+def table_class_name(eng, cls):
+    import ctypes
+    l4 = (ctypes.c_int * 4)()
+    npart = ctypes.c_int()
+    eng.L.qx_class_info(cls, l4, ctypes.byref(npart))
+    return "".join(map(str, l4))
+
+
+def host_cpu():
+    """(logical CPUs, model name) of this host."""
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return os.cpu_count() or 1, model
+
+
 def measured_traffic(workload, kernel):
     """HBM bytes per launch that counter runs recorded for this kernel on this
     workload (profiles/hbm_traffic.txt), else None."""
@@ -147,7 +198,9 @@ def cpu_baseline(mol, basis, tol, eng, seconds):
                 counts[i] += kc
                 ket_lists.setdefault(i, []).append((ki, kc))
     total = counts.sum()
-    nthreads = min(16, os.cpu_count() or 1)
+    # the host cores one GPU's job may use here: OMP_NUM_THREADS is that
+    # share (16 per GPU on the MI355X hosts), else every logical CPU
+    nthreads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
 
     cum = np.cumsum(counts)
 
@@ -198,6 +251,34 @@ def hf_energy_gradient(eng, mol, precision, thr):
     grad = rhf_gradient(eng, aux, precision=precision, fp32_threshold=thr)
     torch.cuda.synchronize()
     return E, aux, grad, t_scf, time.perf_counter() - t0
+
+
+def hf_mixed_vs_fp64(workload_mol, thr):
+    """RHF energy + analytic gradient wall time of one molecule, FP64 and
+    mixed (FP32 recursion below the Schwarz-bound threshold), with the mixed
+    run's energy and gradient error against FP64."""
+    import numpy as np
+    from quax_amd.basis import BasisSet
+    from quax_amd.engine import EriEngine
+    mol = workload_mol
+    basis = BasisSet.build(mol, "even-tempered", pure=True)
+    with EriEngine(mol, basis, ints_tol=1e-12) as eng:
+        n64, n32 = eng.mixed_quartets(thr)
+        S0, _, _ = eng.oei()
+        for prec in (0, 1):
+            eng.jk(S0, precision=prec, fp32_threshold=thr)
+            eng.eri_grad(S0, precision=prec, fp32_threshold=thr)
+        E, aux, g, ts, tg = hf_energy_gradient(eng, mol, "fp64", thr)
+        Em, auxm, gm, tsm, tgm = hf_energy_gradient(eng, mol, "mixed", thr)
+    log(f"[bench] {mol.natom()} atoms ({basis.nbf()} AO): FP64 HF energy + gradient {ts + tg:.2f} s "
+        f"({aux['iterations']} it), mixed @ {thr:g} {tsm + tgm:.2f} s ({auxm['iterations']} it), "
+        f"dE {Em - E:.2e}, max dg {np.abs(gm - g).max():.2e}")
+    return {"natom": mol.natom(), "nbf": basis.nbf(), "fp32_threshold": thr,
+            "fp32_quartet_fraction": n32 / max(n64 + n32, 1),
+            "hf_gradient_wall_s_fp64": ts + tg, "hf_gradient_wall_s_mixed": tsm + tgm,
+            "scf_iterations_fp64": aux["iterations"], "scf_iterations_mixed": auxm["iterations"],
+            "energy_error_mixed": Em - E, "grad_max_error_mixed": float(np.abs(gm - g).max()),
+            "speedup_mixed": (ts + tg) / (tsm + tgm)}
 
 
 def timed_steps(eng, D, steps, warmup, precision, thr, profile):
@@ -290,6 +371,8 @@ def main():
     ap.add_argument("--sweep", default="",
                     help="comma-separated FP32 thresholds: mixed HF energy + gradient error and time at each")
     ap.add_argument("--no-mixed", action="store_true", help="skip every mixed-precision run (FP64-only builds)")
+    ap.add_argument("--no-cluster24", action="store_true",
+                    help="skip the 24-atom mixed-vs-FP64 HF energy + gradient timing")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 512 MiB device copy (HBM-counter calibration for profiler runs)")
     args = ap.parse_args()
@@ -402,11 +485,25 @@ def main():
     slot = int(np.argmax(ms))
     name, fl = table[slot][:2]
     avg_ms = ms[slot] / launches[slot]
-    flops_per_launch = fl * pq[slot] / launches[slot]
+    model = flop_model()
+    # roofline of the dominant kernel: ALGORITHMIC flops (textbook model) per launch
+    alg_per_pq = slot_algorithmic_flops(table, model, eng, slot)
+    flops_per_launch = alg_per_pq * pq[slot] / launches[slot]
+    executed_per_launch = fl * pq[slot] / launches[slot]
     achieved = flops_per_launch / (avg_ms * 1e-3)
     total_flops = sum(table[k][1] * pq[k] for k in range(len(ms)) if launches[k]) / args.steps
+    alg_step = sum(model["".join(map(str, l4))]["jk"] * ne + model["".join(map(str, l4))]["adjoint"] * nd
+                   for (l4, ne, nd, _) in cq)
     kernels = sorted(((float(ms[k]) / args.steps, table[k][0], int(pq[k] // max(launches[k], 1)))
                       for k in range(len(ms)) if launches[k]), reverse=True)[:8]
+
+    hf24 = None
+    if args.workload == "cluster10" and not args.no_cluster24 and not args.no_mixed:
+        # the metric's mixed-vs-FP64 HF-gradient wall time is quoted on the
+        # 24-atom cluster (672 spherical functions)
+        eng.close()
+        hf24 = hf_mixed_vs_fp64(workloads.cluster24(), thr)
+        eng = EriEngine(mol, basis, ints_tol=tol)
 
     cpu = None
     if args.cpu_seconds > 0:
@@ -415,10 +512,12 @@ def main():
             r_e, n_e, t_e = res["eri"]
             r_d, n_d, t_d = res["deri"]
             t_cpu_step = q_eri / r_e + q_deri / r_d
+            ncpu, cpu_model = host_cpu()
             cpu = {"value": (q_eri + q_deri) / t_cpu_step, "unit": "shell-quartets/s", "cores": nthreads,
-                   "kind": "port",
+                   "kind": "port", "host_logical_cpus": ncpu, "host_cpu_model": cpu_model,
                    "sample": f"{n_e} ERI + {n_d} dERI canonical screened quartets drawn uniformly from the same "
-                             f"workload, CPU oracle (C, OpenMP {nthreads} threads): ERI {r_e:.3g}/s, dERI "
+                             f"workload, CPU oracle (C, OpenMP {nthreads} threads; a clarity-first restatement "
+                             f"that allocates per quartet, not Libint): ERI {r_e:.3g}/s, dERI "
                              f"{r_d:.3g}/s, combined over the step's mix"}
         except Exception as e:  # the baseline is reported, never required
             log(f"[bench] cpu baseline failed: {e}")
@@ -442,17 +541,25 @@ def main():
                    "eri_quartets": q_eri, "deri_quartets": q_deri, "streams": args.streams,
                    "one_stream_ms_per_step": dts * 1e3 / args.steps,
                    "eri_kernel_ms": eri_ms, "deri_kernel_ms": deri_ms},
-        "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
+        "roofline": {"bound": "valu", "bound_note": "FP64 vector ALU: the recurrences use no MFMA; MI355X FP64 "
+                                                    "vector peak 78.6 TFLOP/s",
+                     "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64, "traffic": measured_traffic(args.workload, name),
                      "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated)",
                      "kernel": name, "avg_launch_ms": avg_ms,
-                     "flops_per_launch": flops_per_launch, "step_flops": total_flops,
-                     "step_tflops": total_flops / (ms_step * 1e-3) / 1e12},
+                     "flops_per_launch": flops_per_launch, "flops_model": "quax_amd/csrc/flop_model.json",
+                     "executed_flops_per_launch": executed_per_launch,
+                     "step_algorithmic_flops": alg_step,
+                     "step_algorithmic_tflops": alg_step / (ms_step * 1e-3) / 1e12,
+                     "step_algorithmic_frac": alg_step / (ms_step * 1e-3) / PEAK_FP64,
+                     "step_executed_flops": total_flops,
+                     "step_executed_tflops": total_flops / (ms_step * 1e-3) / 1e12},
         "cpu_baseline": cpu,
         "mixed": None if dtm is None else
         {"fp32_threshold": thr, "ms_per_step": dtm * 1e3 / args.steps,
          "value": (q_eri + q_deri) / (dtm / args.steps), "speedup_vs_fp64_step": dt / dtm},
         "hf": hf,
+        "hf_cluster24": hf24,
         "top_kernels_ms_per_step": kernels,
         "class_kernel_ms_per_step": class_times(table, ms, args.steps),
     }
