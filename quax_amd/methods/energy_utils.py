@@ -28,7 +28,8 @@ def partial_tei_transformation(G, Ci, Cj, Ck, Cl, device=None):
     if not torch.cuda.is_available():
         raise _lib.NativeUnavailable("quax_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
     L = _lib.load()
-    device = device or (G.device if hasattr(G, "device") and G.device.type == "cuda" else torch.device("cuda", 0))
+    if device is None:
+        device = G.device if isinstance(G, torch.Tensor) and G.is_cuda else torch.device("cuda", 0)
     G = _dev(torch, G, device)
     Cs = [_dev(torch, C, device) for C in (Ci, Cj, Ck, Cl)]
     if G.dim() != 4:

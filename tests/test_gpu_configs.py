@@ -15,6 +15,7 @@
   against the all-FP64 path, |dE| <= 1e-8 Eh and |dg| <= 1e-7 Eh/bohr.
 """
 import math
+import time
 
 import numpy as np
 import pytest
@@ -37,21 +38,15 @@ def _density(n, seed):
     return X @ X.T / n
 
 
-def _jk_from_store(torch, store, lists, pairs, bf_first, D, chunk=1 << 22):
-    """J_pq = sum_rs (pq|rs) D_rs and K_pq = sum_rs (pr|qs) D_rs from the
-    unique integrals: every stored element is expanded into its 8 index
-    permutations, each weighted 1/2 per coincidence (a = b, c = d, ab = cd of
-    the canonical shell quartet), which counts each element of the full
-    tensor once."""
+def _full_from_store(torch, store, lists, pairs, bf_first, n, chunk=1 << 22):
+    """The full (n, n, n, n) tensor (pq|rs) from the unique integrals: every
+    stored element is written to its 8 index permutations (a plain scatter;
+    coinciding permutations write the same value).  65 GB at n = 300."""
     dev = store.device
-    n = D.shape[0]
-    Dt = torch.as_tensor(D, device=dev).reshape(-1)
-    J = torch.zeros(n * n, dtype=torch.float64, device=dev)
-    K = torch.zeros(n * n, dtype=torch.float64, device=dev)
+    G = torch.zeros(n ** 4, dtype=torch.float64, device=dev)
     pairs_t = torch.as_tensor(pairs, device=dev, dtype=torch.int64)
     bf_t = torch.as_tensor(bf_first, device=dev, dtype=torch.int64)
     for L in lists:
-        la, lb, lc, ld = L["l4"]
         na, nb, nc, nd = (_ncart(x) for x in L["l4"])
         N, nq = na * nb * nc * nd, L["nquart"]
         comp = torch.arange(N, device=dev)
@@ -63,23 +58,30 @@ def _jk_from_store(torch, store, lists, pairs, bf_first, D, chunk=1 << 22):
         for t0 in range(0, nq, step):
             t1 = min(nq, t0 + step)
             p1, p2 = p1all[t0:t1], p2all[t0:t1]
-            sa, sb = pairs_t[p1, 0], pairs_t[p1, 1]
-            sc, sd = pairs_t[p2, 0], pairs_t[p2, 1]
-            f = torch.ones(t1 - t0, dtype=torch.float64, device=dev)
-            f = torch.where(sa == sb, 0.5 * f, f)
-            f = torch.where(sc == sd, 0.5 * f, f)
-            f = torch.where(p1 == p2, 0.5 * f, f)
-            I = (bf_t[sa][None, :] + ci[:, None]).reshape(-1)
-            Jx = (bf_t[sb][None, :] + cj[:, None]).reshape(-1)
-            Kx = (bf_t[sc][None, :] + ck[:, None]).reshape(-1)
-            Lx = (bf_t[sd][None, :] + cl[:, None]).reshape(-1)
+            I = (bf_t[pairs_t[p1, 0]][None, :] + ci[:, None]).reshape(-1)
+            Jx = (bf_t[pairs_t[p1, 1]][None, :] + cj[:, None]).reshape(-1)
+            Kx = (bf_t[pairs_t[p2, 0]][None, :] + ck[:, None]).reshape(-1)
+            Lx = (bf_t[pairs_t[p2, 1]][None, :] + cl[:, None]).reshape(-1)
             idx = L["base"] + comp[:, None] * nq + torch.arange(t0, t1, device=dev)[None, :]
-            v = (store[idx.reshape(-1)].reshape(N, -1) * f[None, :]).reshape(-1)
+            v = store[idx.reshape(-1)]
             for (p, q, r, s) in ((I, Jx, Kx, Lx), (Jx, I, Kx, Lx), (I, Jx, Lx, Kx), (Jx, I, Lx, Kx),
                                  (Kx, Lx, I, Jx), (Lx, Kx, I, Jx), (Kx, Lx, Jx, I), (Lx, Kx, Jx, I)):
-                J.index_add_(0, p * n + q, v * Dt[r * n + s])
-                K.index_add_(0, p * n + r, v * Dt[q * n + s])
-    return J.reshape(n, n), K.reshape(n, n)
+                G[((p * n + q) * n + r) * n + s] = v
+    return G.reshape(n, n, n, n)
+
+
+def _jk_from_full(torch, G, D):
+    """J_pq = sum_rs (pq|rs) D_rs, K_pq = sum_rs (pr|qs) D_rs from the full
+    tensor, with dense products only (no atomics)."""
+    n = D.shape[0]
+    Dt = torch.as_tensor(D, device=G.device, dtype=torch.float64)
+    J = (G.reshape(n * n, n * n) @ Dt.reshape(-1)).reshape(n, n)
+    K = torch.empty((n, n), dtype=torch.float64, device=G.device)
+    Dcol = Dt.unsqueeze(-1).contiguous()
+    for p in range(n):
+        # K[p, q] = sum_r sum_s G[p, r, q, s] D[r, s]
+        K[p] = torch.bmm(G[p], Dcol).sum(0).reshape(-1)
+    return J, K
 
 
 def _e2(torch, eng, D):
@@ -88,19 +90,26 @@ def _e2(torch, eng, D):
     return float(torch.sum(Dt * (2 * J - K)))
 
 
+def _say(t0, msg):
+    print(f"  [{time.time() - t0:6.1f} s] {msg}", flush=True)
+
+
 def test_cluster10_full_size(gpu):
     import torch
     from quax_amd.engine import EriEngine
     from quax_amd.methods.hartree_fock import restricted_hartree_fock
+    t0 = time.time()
     mol = workloads.cluster10()
     bs = BasisSet.build(mol, "even-tempered")            # Cartesian: 300 functions
     ob = oracle.OracleBasis(mol, bs)
     bf_first = np.concatenate([[0], np.cumsum([_ncart(s.l) for s in bs.shells])]).astype(np.int64)
     with EriEngine(mol, bs, ints_tol=1e-12) as eng:
-        assert eng.nbf == 300
+        n = eng.nbf
+        assert n == 300
         nq_total = eng.num_quartets()
         store = eng.store()
         lists, pairs = eng.store_map()
+        _say(t0, f"stored {store.numel()} unique integrals of {nq_total} quartets")
         assert len(lists) == 21
         assert sum(L["nquart"] for L in lists) == nq_total   # single-primitive shells
         # -- per-integral parity on a seeded sample of every class list
@@ -118,17 +127,24 @@ def test_cluster10_full_size(gpu):
                 ref = ob.shell_quartet(int(a), int(b), int(c), int(d)).reshape(-1)
                 worst = max(worst, float(np.abs(vals[:, j] - ref).max()))
                 checked += 1
+        _say(t0, f"{checked} sampled quartets vs oracle: max |d| {worst:.2e}")
         assert checked >= 2000
         assert worst < 1e-12, worst
         # -- converged RHF density (direct J/K), then J/K and E from the store
         E, aux = restricted_hartree_fock(eng, mol.nelectron(), {"spectral_shift": False, "diis": True,
                                                                 "conv": 1e-10, "maxit": 200},
                                          return_aux_data=True)
+        _say(t0, f"RHF E = {E:.10f} in {aux['iterations']} iterations")
         D = aux["D"].cpu().numpy()
         Jd, Kd = (x.cpu().numpy() for x in eng.jk(D))
-        Js, Ks = (x.cpu().numpy() for x in _jk_from_store(torch, store, lists, pairs, bf_first, D))
+        G = _full_from_store(torch, store, lists, pairs, bf_first, n)
         del store
         torch.cuda.empty_cache()
+        _say(t0, "full tensor scattered from the store")
+        Js, Ks = (x.cpu().numpy() for x in _jk_from_full(torch, G, D))
+        del G
+        torch.cuda.empty_cache()
+        _say(t0, f"J/K from the store: max |dJ| {np.abs(Jd - Js).max():.2e}, max |dK| {np.abs(Kd - Ks).max():.2e}")
         assert np.abs(Jd - Js).max() < 1e-11 * max(1.0, np.abs(Js).max())
         assert np.abs(Kd - Ks).max() < 1e-11 * max(1.0, np.abs(Ks).max())
         H = aux["H"].cpu().numpy()
@@ -151,6 +167,7 @@ def test_cluster10_full_size(gpu):
             with EriEngine(m, bs, ints_tol=1e-12) as e1:
                 e.append(_e2(torch, e1, D))
         fd = (e[0] - e[1]) / (2 * h)
+        _say(t0, f"dE2/dx[{k}]: adjoint {g[k]:.10f}, central difference {fd:.10f}")
         assert abs(g[k] - fd) < 1e-6 * max(1.0, abs(fd)), (k, g[k], fd)
     assert np.isfinite(e2_0)
 
@@ -179,6 +196,7 @@ def test_subset4_rhf_gradient_vs_finite_differences(gpu):
         xp[k] += h
         xm[k] -= h
         fd[k] = (energy(xp) - energy(xm)) / (2 * h)
+        print(f"  subset4 dE/dx[{k}]: analytic {g[k]:.10f}, central difference {fd[k]:.10f}", flush=True)
     assert np.abs(g - fd).max() < 1e-7, np.abs(g - fd).max()
     assert np.abs(g.reshape(-1, 3).sum(0)).max() < 1e-9
 
@@ -194,6 +212,7 @@ def test_cluster24_mixed_precision_vs_fp64(gpu):
         nq = eng.num_quartets()
         E64, aux64 = restricted_hartree_fock(eng, mol.nelectron(), dict(opts), return_aux_data=True)
         g64 = rhf_gradient(eng, aux64)
+        print(f"cluster24 FP64: E = {E64:.10f} in {aux64['iterations']} iterations", flush=True)
         # the headline threshold from a cold start, the others warm-started
         # from the FP64 density
         for thr, warm in ((1e-8, False), (1e-10, True), (1e-6, True)):
