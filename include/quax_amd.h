@@ -117,6 +117,18 @@ long long qx_generate_multi_index_lookup(int nparams, int deriv_order, int* out,
 int qx_jk_build(qx_ctx* ctx, const double* D, double* J, double* K, int precision, double fp32_threshold,
                 void* stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): qx_jk_build with precision QX_PREC_FP64 that allocates
+ * nothing.  Its J'/K' work matrices live in a file-scope device array of
+ * qx_jk_noalloc_capacity() doubles (24 MiB, held by every GPU the library
+ * runs on until the process ends): 2 n_cart^2 doubles for Cartesian bases,
+ * 5 n_cart^2 for spherical ones; larger contexts return QX_EINVAL with a
+ * message.  All kernels run in turn on `stream`.  Calls that overlap on one
+ * GPU share the array and race; the array is cleared on every call. */
+int qx_jk_build_noalloc(qx_ctx* ctx, const double* D, double* J, double* K, void* stream);
+long long qx_jk_noalloc_capacity(void);
+
 /* Adjoint two-electron gradient (no reference counterpart: the reference
  * differentiates forward through eri_deriv_core, tei.py:25-34):
  * grad[3*natom] += sum_pqrs (2 D_pq D_rs - D_pr D_qs) d(pq|rs)/dR.

@@ -51,6 +51,8 @@ SIGNATURES = {
     "qx_how_many_derivs": (_ll, [_i, _i]),
     "qx_generate_multi_index_lookup": (_ll, [_i, _i, _ip, _ll]),
     "qx_jk_build": (_i, [_vp, _vp, _vp, _vp, _i, ctypes.c_double, _vp]),
+    "qx_jk_build_noalloc": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "qx_jk_noalloc_capacity": (_ll, []),
     "qx_eri_grad": (_i, [_vp, _vp, _vp, _i, ctypes.c_double, _vp]),
     "qx_oei": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "qx_oei_grad": (_i, [_vp, _vp, _vp, _vp, _vp]),

@@ -103,6 +103,9 @@ struct qx_ctx {
   // Cartesian D, symmetrised J and K); ws_ev orders a pass after the previous
   // pass that used them, whatever stream each ran on
   double* ws = nullptr;
+  // device address of the file-scope scratch array on this context's device
+  // (qx_jk_build_noalloc)
+  double* scratch = nullptr;
   hipEvent_t ws_ev = nullptr;
   bool ws_used = false;
 

@@ -10,6 +10,7 @@
 #include <exception>
 #include <functional>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <numeric>
 #include <stdexcept>
@@ -641,6 +642,32 @@ void setup_shell_tables(qx_ctx* c) {
   s.sph = dupload(c, blk);
 }
 
+// Work matrices of qx_jk_build_noalloc: one file-scope device array per GPU
+// (each device holds its own copy until the process ends), 3 Mi doubles =
+// 24 MiB, enough for 5 n_cart^2 matrices up to n_cart = 792 (spherical
+// bases) or 2 n_cart^2 up to n_cart = 1254 (Cartesian).  Host code never
+// uses the array's name as an address: qx_initialize looks up its device
+// address once per device and keeps it in g_scratch_ptr.
+#define QX_NOALLOC_DOUBLES (3ull << 20)
+static __device__ __align__(256) double scratch[QX_NOALLOC_DOUBLES];
+
+#define QX_MAX_DEVICES 64
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[QX_MAX_DEVICES] = {};
+
+// Device address of `scratch` on `device`, which must be current.
+double* scratch_address(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  QX_REQUIRE(device >= 0 && device < QX_MAX_DEVICES, QX_EINVAL, "device index beyond the scratch table");
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    QX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    QX_REQUIRE(p != nullptr, QX_EHIP, "no device address for the scratch array");
+    g_scratch_ptr[device] = static_cast<double*>(p);
+  }
+  return g_scratch_ptr[device];
+}
+
 thread_local std::string g_init_error;
 
 int fail(qx_ctx* c, int code, const char* msg) noexcept {
@@ -719,6 +746,7 @@ static void init_body(qx_ctx* c, int natom, const double* xyz_bohr, const double
     QX_REQUIRE(device >= 0 && device < ndev, QX_EINVAL, "device index out of range");
     QX_HIP(hipSetDevice(device));
     c->device = device;
+    c->scratch = scratch_address(device);
     if (atom_fragment) {
       for (int a = 0; a < natom; ++a) {
         QX_REQUIRE(atom_fragment[a] >= 0, QX_EINVAL, "fragment ids must be non-negative");
@@ -1028,6 +1056,61 @@ int qx_jk_build(qx_ctx* c, const double* D, double* J, double* K, int precision,
     return QX_OK;
   });
 }
+
+// qx_jk_build (FP64) without any device allocation: the J'/K' work
+// matrices live in the scratch array, every class kernel runs in turn on the
+// caller's stream (no side streams, events or timing), and the work matrices
+// are cleared by a fill kernel on every call.  Calls that overlap on one GPU
+// share the array and race.
+int qx_jk_build_noalloc(qx_ctx* c, const double* D, double* J, double* K, void* stream) {
+  if (!c || !D || !J || !K) return QX_EINVAL;
+  return guarded(c, [&]() -> int {
+    QX_HIP(hipSetDevice(c->device));
+    hipStream_t s = (hipStream_t)stream;
+    const size_t nc = c->ncart, n2 = nc * nc;
+    const size_t need = (c->pure ? 5 : 2) * n2;
+    if (need > QX_NOALLOC_DOUBLES)
+      throw QxError{QX_EINVAL, "jk_build_noalloc: " + std::to_string(need) + " doubles of work matrices (" +
+                                   std::to_string(nc) + " Cartesian functions) exceed the " +
+                                   std::to_string(QX_NOALLOC_DOUBLES) + "-double scratch array"};
+    QX_REQUIRE(c->scratch != nullptr, QX_EHIP, "jk_build_noalloc: scratch array not set up");
+    double* w = c->scratch;
+    double *Jp = w, *Kp = w + n2;
+    const double* Dcart = D;
+    if (c->pure) {
+      check_launch(qxk_ao_to_cart_mat(&c->shells, D, w + 2 * n2, s), "ao_to_cart");
+      Dcart = w + 2 * n2;
+    }
+    check_launch(qxk_fill(Jp, (long long)n2, 0.0, s), "fill");
+    check_launch(qxk_fill(Kp, (long long)n2, 0.0, s), "fill");
+    QxEpiDev e{};
+    e.epi = QX_EPI_JK; e.nbf = (int)nc; e.natom = c->natom; e.coord = -1; e.D = Dcart; e.J = Jp; e.K = Kp;
+    e.krows = 1;
+    for (const auto& wh : c->work) {
+      if (wh.w.nquart <= 0) continue;
+      const QxClassEntry& ent = qx_class_table[wh.cls];
+      for (int k = 0; k < ent.njk; ++k) {
+        QxArgs a;
+        a.pr = c->pr; a.pp = c->pp; a.w = wh.w; a.e = e; a.boys = c->d_boys;
+        QxLaunch L{0, s};
+        check_launch(ent.jk[k](&a, &L), "jk_build_noalloc");
+      }
+    }
+    if (c->pure) {
+      double *Js = w + 3 * n2, *Ks = w + 4 * n2;
+      check_launch(qxk_sym_half(Jp, Js, (int)nc, s), "sym");
+      check_launch(qxk_sym_half(Kp, Ks, (int)nc, s), "sym");
+      check_launch(qxk_cart_to_ao_mat(&c->shells, Js, J, s), "cart_to_ao");
+      check_launch(qxk_cart_to_ao_mat(&c->shells, Ks, K, s), "cart_to_ao");
+    } else {
+      check_launch(qxk_sym_half(Jp, J, (int)nc, s), "sym");
+      check_launch(qxk_sym_half(Kp, K, (int)nc, s), "sym");
+    }
+    return QX_OK;
+  });
+}
+
+long long qx_jk_noalloc_capacity(void) { return (long long)QX_NOALLOC_DOUBLES; }
 
 int qx_eri_grad(qx_ctx* c, const double* D, double* grad, int precision, double thr, void* stream) {
   if (!c || !D || !grad) return QX_EINVAL;
