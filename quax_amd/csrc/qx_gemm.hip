@@ -10,9 +10,10 @@
 //
 // Kernel: 64 x 64 output tile per 256-thread workgroup (four 64-wide
 // wavefronts, 32 x 32 each as 2 x 2 tiles of v_mfma_f64_16x16x4f64), K in
-// steps of 32 staged through LDS.  MFMA operand layout (16x16x4, FP64):
-// lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16]; accumulator
-// element r of lane l is D[4 (l / 16) + r][l % 16].
+// steps of 32 staged through LDS.  MFMA operand layout (16x16x4, FP64,
+// checked on gfx950 with a one-wavefront probe): lane l supplies
+// A[l % 16][l / 16] and B[l / 16][l % 16]; accumulator element r of lane l is
+// D[4 r + l / 16][l % 16] (rows interleaved across the four 16-lane groups).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -88,7 +89,7 @@ __global__ __launch_bounds__(GM_THREADS) void qx_dgemm_tt_kernel(int M, long lon
     for (int tb = 0; tb < 2; ++tb)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const long long gi = i0 + wr * 32 + ta * 16 + 4 * (lane >> 4) + r;
+        const long long gi = i0 + wr * 32 + ta * 16 + 4 * r + (lane >> 4);
         const long long gj = j0 + wc * 32 + tb * 16 + (lane & 15);
         if (gi < M && gj < N) out[gi * ldo + gj] = acc[ta][tb][r];
       }
