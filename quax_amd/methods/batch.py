@@ -10,7 +10,7 @@ from one small context per molecule (they couple every nucleus).  The SCF
 runs in lock step: one batched J/K build, then per molecule the same
 update as restricted_hartree_fock (hartree_fock.py:25-99; DIIS on the
 orbital gradient of the density that built F), with every molecule's
-orthogonalised Fock matrix diagonalised in one padded batched eigh.
+orthogonalised Fock matrix diagonalised in one padded batched host eigh.
 Converged molecules keep their density while the others iterate.
 """
 from __future__ import annotations
@@ -109,7 +109,11 @@ def batched_rhf(molecules, basis_name="even-tempered", pure=True, ints_tol=1e-12
                 F = diis(k, F, err)
             Fp[k].copy_(pad)
             Fp[k, :n, :n] = A[k].T @ F @ A[k]
-        eps, C2 = torch.linalg.eigh(Fp)
+        # all molecules' Fock matrices in one batched LAPACK call on the host
+        # (the padded device eigh is not used: see the incident record of
+        # the batched driver)
+        w_, v_ = np.linalg.eigh(Fp.cpu().numpy())
+        eps, C2 = torch.as_tensor(w_, device=dev), torch.as_tensor(v_, device=dev)
         En, dr = Enew.cpu().numpy(), drms.cpu().numpy()
         for k in range(B):
             if done[k]:

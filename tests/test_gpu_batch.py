@@ -75,8 +75,6 @@ def _well_behaved():
             Molecule(["C", "O"], [[0.0, 0.0, 0.0], [0.0, 0.0, 2.13]])]
 
 
-@pytest.mark.skip(reason="aborted (SIGABRT, no GPU fault, no message) when run after the rest of the GPU "
-                         "suite, though it passed alone; cause not yet found, kept off the GPU until it is")
 def test_batched_rhf_matches_single_molecule_runs(gpu):
     from quax_amd.engine import EriEngine
     from quax_amd.methods.batch import batched_rhf
