@@ -89,13 +89,17 @@ int qx_num_quartets(const qx_ctx* ctx, long long* nquartets);
 int qx_compute_2e_int(qx_ctx* ctx, double* out, void* stream);
 
 /* eri_deriv_core(deriv_order) (libint_interface.cc:2582-2779): out holds
- * how_many_derivs(natom, order) * nbf^4 doubles.  deriv_order 1 only
- * (QX_EUNSUPPORTED otherwise).  Quartets on a single atom are skipped. */
+ * how_many_derivs(natom, order) * nbf^4 doubles, slices in
+ * generate_multi_index_lookup(3*natom, order) order.  deriv_order 1 and 2
+ * (QX_EUNSUPPORTED above; LIBINT2_MAX_DERIV_ORDER of this build is 2).
+ * Quartets on a single atom are skipped.  Order 2 runs the first-derivative
+ * kernels in forward mode along each nuclear coordinate. */
 int qx_eri_deriv_core(qx_ctx* ctx, int deriv_order, double* out, void* stream);
 
 /* compute_2e_deriv("eri", 0., deriv_vec) (libint_interface.cc:1097-1335):
  * one nbf^4 slice.  len must equal 3*natom (QX_EINVAL otherwise); total order
- * 0 or 1 (QX_EUNSUPPORTED above). */
+ * 0, 1 or 2 (QX_EUNSUPPORTED above).  Order 0 is the integral tensor with
+ * single-atom quartets left out, as the reference's deriv-0 engine gives. */
 int qx_compute_2e_deriv(qx_ctx* ctx, const int* deriv_vec, int len, double* out, void* stream);
 
 /* how_many_derivs (utils.h:131-140): C(3*natom + order - 1, order). */
@@ -134,6 +138,22 @@ long long qx_jk_noalloc_capacity(void);
  * grad[3*natom] += sum_pqrs (2 D_pq D_rs - D_pr D_qs) d(pq|rs)/dR.
  * Integrals are recomputed, the derivative tensor is never stored. */
 int qx_eri_grad(qx_ctx* ctx, const double* D, double* grad, int precision, double fp32_threshold, void* stream);
+
+/* Hessian-vector products by forward-over-reverse (no reference
+ * counterpart: the reference forms Hessians with jacfwd(jacfwd) over
+ * eri_deriv_core(2), core.py:89-91).  v: 3*natom device vector (a geometry
+ * direction); D, Pd, Wd: nbf x nbf device buffers.  The adjoint kernels run
+ * in forward mode along v, recomputing integrals; nothing n^4 is stored.
+ *   qx_eri_hess_dir: out[3N] += sum_pqrs (2 D_pq D_rs - D_pr D_qs)
+ *                    d/dv d(pq|rs)/dR.
+ *   qx_jk_deriv_dir: J_pq = sum_rs d/dv (pq|rs) D_rs, K_pq = sum_rs
+ *                    d/dv (pr|qs) D_rs (overwritten), the skeleton derivative
+ *                    Fock matrix of the coupled-perturbed equations.
+ *   qx_oei_hess_dir: grad[3N] += sum_pq Pd_pq d/dv d(T+V)_pq/dR +
+ *                    Wd_pq d/dv dS_pq/dR. */
+int qx_eri_hess_dir(qx_ctx* ctx, const double* D, const double* v, double* out, void* stream);
+int qx_jk_deriv_dir(qx_ctx* ctx, const double* D, const double* v, double* J, double* K, void* stream);
+int qx_oei_hess_dir(qx_ctx* ctx, const double* Pd, const double* Wd, const double* v, double* grad, void* stream);
 
 /* One-electron integrals S, T, V (nbf x nbf device buffers); Libint
  * overlap / kinetic / nuclear operators as compute_1e_int uses them

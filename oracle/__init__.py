@@ -50,12 +50,16 @@ def lib():
         L.or_get_coefs.argtypes = [ctypes.c_void_p, _dp]
         L.or_shell_quartet.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]
         L.or_shell_quartet_deriv1.argtypes = L.or_shell_quartet.argtypes
+        L.or_shell_quartet_deriv2.argtypes = L.or_shell_quartet.argtypes
         L.or_schwarz.argtypes = [ctypes.c_void_p, ctypes.c_double, _dp, _ip]
         L.or_schwarz.restype = ctypes.c_int
         L.or_compute_2e_int.argtypes = [ctypes.c_void_p, ctypes.c_double, _dp]
         L.or_eri_deriv_core1.argtypes = [ctypes.c_void_p, ctypes.c_double, _dp]
         L.or_compute_2e_deriv1.argtypes = [ctypes.c_void_p, ctypes.c_double, _ip, _dp]
         L.or_compute_2e_deriv1.restype = ctypes.c_int
+        L.or_eri_deriv_core2.argtypes = [ctypes.c_void_p, ctypes.c_double, _dp]
+        L.or_compute_2e_deriv2.argtypes = [ctypes.c_void_p, ctypes.c_double, _ip, _dp]
+        L.or_compute_2e_deriv2.restype = ctypes.c_int
         L.or_quartet_batch.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip, ctypes.c_int, _dp,
                                        ctypes.POINTER(ctypes.c_longlong)]
         L.or_quartet_batch.restype = ctypes.c_double
@@ -122,6 +126,13 @@ class OracleBasis:
         lib().or_shell_quartet_deriv1(self.h, a, b, c, d, _p(out))
         return out
 
+    def shell_quartet_deriv2(self, a, b, c, d):
+        """78 blocks: centre-coordinate pairs i <= j of 0..11, lexicographic."""
+        n = [ncart(self.l[s]) for s in (a, b, c, d)]
+        out = np.zeros([78] + n)
+        lib().or_shell_quartet_deriv2(self.h, a, b, c, d, _p(out))
+        return out
+
     def schwarz(self, tol):
         ns = len(self.l)
         vals = np.zeros((ns, ns))
@@ -148,6 +159,22 @@ class OracleBasis:
         rc = lib().or_compute_2e_deriv1(self.h, tol, _p(dv, _ip), _p(out))
         if rc != 0:
             raise ValueError("oracle supports first-order deriv_vec only")
+        return out
+
+    def eri_deriv_core2(self, tol):
+        n = self.nbf
+        m = 3 * self.natom
+        out = np.zeros((m * (m + 1) // 2, n, n, n, n))
+        lib().or_eri_deriv_core2(self.h, tol, _p(out))
+        return out
+
+    def compute_2e_deriv2(self, tol, deriv_vec):
+        n = self.nbf
+        dv = np.ascontiguousarray(deriv_vec, dtype=np.int32)
+        out = np.zeros((n, n, n, n))
+        rc = lib().or_compute_2e_deriv2(self.h, tol, _p(dv, _ip), _p(out))
+        if rc != 0:
+            raise ValueError("compute_2e_deriv2 needs a second-order deriv_vec")
         return out
 
     def quartet_batch(self, quartets, nthreads=1, want=False, deriv=False):

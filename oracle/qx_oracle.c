@@ -15,16 +15,18 @@
  *   - compute_2e_int("eri") (libint_interface.cc:486-647): bra pairs x ket
  *     pairs, skip when Q_ab*Q_cd < tol^2, scatter into 4/2/1 permutations.
  *   - compute_2e_deriv("eri", deriv_vec) (libint_interface.cc:1097-1335) and
- *     eri_deriv_core(1) (libint_interface.cc:2582-2779): skip single-atom
- *     quartets, sum the shell-centre derivative buffers of the centres that sit
- *     on the differentiated atom.
+ *     eri_deriv_core(1 and 2) (libint_interface.cc:2582-2779): skip
+ *     single-atom quartets, sum the shell-centre derivative buffers of the
+ *     centres that sit on the differentiated atom(s) -- at order 2 every
+ *     (centre, centre) combination, looked up among the 78 buffers.
  *   - The arithmetic Libint supplies (the un-vendored libint2 Engine, any
  *     version: libint_interface.cc:501, 561, 2600, 2648): contracted Cartesian
  *     Coulomb integrals by the Head-Gordon-Pople scheme -- Obara-Saika
  *     vertical recurrence on primitive quartets, contraction, then horizontal
  *     recurrence -- with derivatives from d/dA_i phi_a = 2 alpha phi_{a+1_i} -
- *     a_i phi_{a-1_i} evaluated for all four centres explicitly (no
- *     translational-invariance shortcut, so invariance is a test, not an input).
+ *     a_i phi_{a-1_i} (applied twice for second derivatives) evaluated for all
+ *     four centres explicitly (no translational-invariance shortcut, so
+ *     invariance is a test, not an input).
  *   - One-electron S, T, V and their first nuclear derivatives (Libint
  *     Operator::overlap/kinetic/nuclear as used by oei_deriv_core,
  *     libint_interface.cc:2403-2579), needed for RHF energies and gradients.
@@ -539,6 +541,186 @@ int or_compute_2e_deriv1(const or_basis* b, double tol, const int* deriv_vec, do
     for (int i = 0; i < ncoord; ++i) { order += deriv_vec[i]; if (deriv_vec[i]) coord = i; }
     if (order != 1) return -1;
     return deriv1_common(b, tol, coord, out);
+}
+
+/* ---------------------------------------------------- second derivatives */
+/* d/dA_x phi_n = 2 alpha phi_{n+1_x} - n_x phi_{n-1_x} applied twice, per
+ * primitive quartet: each of the up-to-four terms of a second derivative is
+ * a primitive integral of a class with shifted angular momenta, taken from
+ * the HRR of one VRR table that covers every shift (bra and ket sums raised
+ * by 2).  Written for clarity: the shifted classes are evaluated per
+ * primitive quartet and cached by shift. */
+typedef struct { double c; int g[4]; } or_term;
+
+/* expand d/dX_x (centre X, axis x) of the product of 4 functions given by
+ * component indices g[4] into at most 2 terms */
+static int or_apply(const or_term* t, int X, int x, const double* ex2, or_term* out) {
+    int k = 0;
+    int gu = g_shift(t->g[X], x, +1);
+    out[k] = *t; out[k].c = t->c * ex2[X]; out[k].g[X] = gu; ++k;
+    int nx = g_comp[t->g[X]][x];
+    if (nx > 0) {
+        out[k] = *t; out[k].c = -t->c * nx; out[k].g[X] = g_shift(t->g[X], x, -1); ++k;
+    }
+    return k;
+}
+
+static int ltot(int g) { return g_comp[g][0] + g_comp[g][1] + g_comp[g][2]; }
+
+/* Second derivatives of a contracted shell quartet with respect to pairs of
+ * the 12 centre coordinates (centre-major, then x, y, z).  out holds 78
+ * blocks of na*nb*nc*nd, one per pair i <= j in lexicographic order -- the
+ * order of generate_multi_index_lookup(12, 2) that indexes Libint's deriv-2
+ * buffers (libint_interface.cc:2587). */
+void or_shell_quartet_deriv2(const or_basis* b, int sa, int sb, int sc, int sd, double* out) {
+    init_tables();
+    int ls[4] = {b->l[sa], b->l[sb], b->l[sc], b->l[sd]};
+    int sh[4] = {sa, sb, sc, sd};
+    int Le = ls[0] + ls[1] + 2, Lf = ls[2] + ls[3] + 2;
+    const int NE = coff(Le + 1), NF = coff(Lf + 1);
+    const int nn[4] = {ncart(ls[0]), ncart(ls[1]), ncart(ls[2]), ncart(ls[3])};
+    const int nblk = nn[0] * nn[1] * nn[2] * nn[3];
+    const double* Cn[4] = {ctr(b, sa), ctr(b, sb), ctr(b, sc), ctr(b, sd)};
+    double* V = (double*)malloc(sizeof(double) * (Le + Lf + 1) * NE * NF);
+    double* X = (double*)malloc(sizeof(double) * NE * NF);
+    /* cache of shifted-class blocks: shift d in -2..2 per centre -> 625 keys */
+    double* cache[625];
+    memset(cache, 0, sizeof(cache));
+    double AB[3], CD[3];
+    for (int i = 0; i < 3; ++i) { AB[i] = Cn[0][i] - Cn[1][i]; CD[i] = Cn[2][i] - Cn[3][i]; }
+    memset(out, 0, sizeof(double) * 78 * nblk);
+    for (int i = b->poff[sa]; i < b->poff[sa + 1]; ++i)
+    for (int j = b->poff[sb]; j < b->poff[sb + 1]; ++j)
+    for (int k = b->poff[sc]; k < b->poff[sc + 1]; ++k)
+    for (int m = b->poff[sd]; m < b->poff[sd + 1]; ++m) {
+        const double w = b->coef[i] * b->coef[j] * b->coef[k] * b->coef[m];
+        const double ex2[4] = {2 * b->alpha[i], 2 * b->alpha[j], 2 * b->alpha[k], 2 * b->alpha[m]};
+        vrr_prim(Le, Lf, b->alpha[i], Cn[0], b->alpha[j], Cn[1], b->alpha[k], Cn[2], b->alpha[m], Cn[3], V, X);
+        for (int q = 0; q < 625; ++q) { free(cache[q]); cache[q] = NULL; }
+        int pair = 0;
+        for (int I = 0; I < 12; ++I)
+        for (int J = I; J < 12; ++J, ++pair) {
+            const int cI = I / 3, xI = I % 3, cJ = J / 3, xJ = J % 3;
+            for (int e = 0; e < nblk; ++e) {
+                int rem = e, idx[4];
+                for (int t = 3; t >= 0; --t) { idx[t] = rem % nn[t]; rem /= nn[t]; }
+                or_term t0;
+                t0.c = 1.0;
+                for (int t = 0; t < 4; ++t) t0.g[t] = coff(ls[t]) + idx[t];
+                or_term t1[2], t2[2];
+                int k1 = or_apply(&t0, cI, xI, ex2, t1);
+                double acc = 0.0;
+                for (int u = 0; u < k1; ++u) {
+                    int k2 = or_apply(&t1[u], cJ, xJ, ex2, t2);
+                    for (int v = 0; v < k2; ++v) {
+                        int l2[4], key = 0;
+                        for (int t = 0; t < 4; ++t) { l2[t] = ltot(t2[v].g[t]); key = key * 5 + (l2[t] - ls[t] + 2); }
+                        if (!cache[key]) {
+                            size_t nb2 = (size_t)ncart(l2[0]) * ncart(l2[1]) * ncart(l2[2]) * ncart(l2[3]);
+                            cache[key] = (double*)malloc(sizeof(double) * nb2);
+                            hrr_quartet(X, NE, NF, l2[0], l2[1], l2[2], l2[3], AB, CD, cache[key]);
+                        }
+                        int o = 0;
+                        for (int t = 0; t < 4; ++t) o = o * ncart(l2[t]) + (t2[v].g[t] - coff(l2[t]));
+                        acc += t2[v].c * cache[key][o];
+                    }
+                }
+                out[(size_t)pair * nblk + e] += w * acc;
+            }
+        }
+    }
+    for (int q = 0; q < 625; ++q) free(cache[q]);
+    free(V); free(X);
+    (void)sh;
+}
+
+/* Shared body of eri_deriv_core(2) and compute_2e_deriv at order 2, as
+ * libint_interface.cc:2582-2779 (and 1097-1335) do it: for each unique pair
+ * (x <= y) of nuclear coordinates, the buffers of every (centre on atom(x),
+ * centre on atom(y)) combination -- sorted, looked up in the 78-buffer
+ * table -- are summed and scattered into the 4/2/1 permutations; quartets
+ * with all four centres on one atom are skipped.  want < 0: every pair,
+ * out[(3N(3N+1)/2) * n^4]; otherwise only that pair index, out[n^4]. */
+static int deriv2_common(const or_basis* b, double tol, int want, double* out) {
+    int ns = b->nshell;
+    size_t n = b->nbf;
+    size_t n4 = n * n * n * n;
+    int ncoord = 3 * b->natom;
+    int npair = ncoord * (ncoord + 1) / 2;
+    double* vals = (double*)malloc(sizeof(double) * ns * ns);
+    int* pairs = (int*)malloc(sizeof(int) * ns * (ns + 1));
+    int np = or_schwarz(b, tol, vals, pairs);
+    double tol2 = tol * tol;
+    int (*plut)[2] = malloc(sizeof(int[2]) * npair);
+    for (int x = 0, q = 0; x < ncoord; ++x)
+        for (int y = x; y < ncoord; ++y, ++q) { plut[q][0] = x; plut[q][1] = y; }
+    int bidx[12][12];
+    for (int I = 0, q = 0; I < 12; ++I)
+        for (int J = I; J < 12; ++J, ++q) bidx[I][J] = bidx[J][I] = q;
+    memset(out, 0, sizeof(double) * (want < 0 ? npair : 1) * n4);
+    #pragma omp parallel for schedule(dynamic)
+    for (int ib = 0; ib < np; ++ib) {
+        int p1 = pairs[2 * ib], p2 = pairs[2 * ib + 1];
+        int n1 = ncart(b->l[p1]), n2 = ncart(b->l[p2]);
+        size_t bf1 = b->bf[p1], bf2 = b->bf[p2];
+        for (int ik = 0; ik < np; ++ik) {
+            int p3 = pairs[2 * ik], p4 = pairs[2 * ik + 1];
+            if (vals[p1 * ns + p2] * vals[p3 * ns + p4] < tol2) continue;
+            int at[4] = {b->atom[p1], b->atom[p2], b->atom[p3], b->atom[p4]};
+            if (at[0] == at[1] && at[0] == at[2] && at[0] == at[3]) continue;
+            int n3 = ncart(b->l[p3]), n4s = ncart(b->l[p4]);
+            size_t bf3 = b->bf[p3], bf4 = b->bf[p4];
+            int nblk = n1 * n2 * n3 * n4s;
+            double* buf = (double*)malloc(sizeof(double) * 78 * nblk);
+            or_shell_quartet_deriv2(b, p1, p2, p3, p4, buf);
+            for (int q = 0; q < npair; ++q) {
+                if (want >= 0 && q != want) continue;
+                int x = plut[q][0], y = plut[q][1];
+                size_t off = want >= 0 ? 0 : (size_t)q * n4;
+                for (int c1 = 0; c1 < 4; ++c1) {
+                    if (at[c1] != x / 3) continue;
+                    for (int c2 = 0; c2 < 4; ++c2) {
+                        if (at[c2] != y / 3) continue;
+                        const double* blk = buf + (size_t)bidx[3 * c1 + x % 3][3 * c2 + y % 3] * nblk;
+                        for (int f1 = 0, idx = 0; f1 < n1; ++f1)
+                            for (int f2 = 0; f2 < n2; ++f2)
+                                for (int f3 = 0; f3 < n3; ++f3)
+                                    for (int f4 = 0; f4 < n4s; ++f4, ++idx) {
+                                        size_t i = bf1 + f1, j = bf2 + f2, k = bf3 + f3, l = bf4 + f4;
+                                        double* e0 = &out[off + ((i * n + j) * n + k) * n + l];
+                                        double v = *e0 + blk[idx];
+                                        *e0 = v;
+                                        if (p1 != p2) out[off + ((j * n + i) * n + k) * n + l] = v;
+                                        if (p3 != p4) out[off + ((i * n + j) * n + l) * n + k] = v;
+                                        if (p1 != p2 && p3 != p4) out[off + ((j * n + i) * n + l) * n + k] = v;
+                                    }
+                    }
+                }
+            }
+            free(buf);
+        }
+    }
+    free(vals); free(pairs); free(plut);
+    return 0;
+}
+
+int or_eri_deriv_core2(const or_basis* b, double tol, double* out) {
+    return deriv2_common(b, tol, -1, out);
+}
+
+/* compute_2e_deriv("eri", 0., deriv_vec) with sum(deriv_vec) == 2. Returns
+ * -1 for any other order. */
+int or_compute_2e_deriv2(const or_basis* b, double tol, const int* deriv_vec, double* out) {
+    int ncoord = 3 * b->natom, order = 0, c[2] = {-1, -1}, k = 0;
+    for (int i = 0; i < ncoord; ++i) {
+        order += deriv_vec[i];
+        for (int r = 0; r < deriv_vec[i] && k < 2; ++r) c[k++] = i;
+    }
+    if (order != 2) return -1;
+    int q = 0;
+    for (int x = 0; x < c[0]; ++x) q += ncoord - x;
+    q += c[1] - c[0];
+    return deriv2_common(b, tol, q, out);
 }
 
 /* Evaluate a list of shell quartets (bench CPU baseline and spot checks).

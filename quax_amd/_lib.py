@@ -54,6 +54,9 @@ SIGNATURES = {
     "qx_jk_build_noalloc": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "qx_jk_noalloc_capacity": (_ll, []),
     "qx_eri_grad": (_i, [_vp, _vp, _vp, _i, ctypes.c_double, _vp]),
+    "qx_eri_hess_dir": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "qx_jk_deriv_dir": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "qx_oei_hess_dir": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "qx_oei": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "qx_oei_grad": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "qx_oei_deriv": (_i, [_vp, _vp, _vp, _vp, _vp]),

@@ -14,7 +14,7 @@ import numpy as np
 
 from .basis import BasisSet
 from .engine import EriEngine
-from .methods.hartree_fock import check_options, restricted_hartree_fock, rhf_gradient
+from .methods.hartree_fock import check_options, restricted_hartree_fock, rhf_gradient, rhf_hessian_vector_product
 
 
 def _setup(molecule, basis_name, options):
@@ -37,17 +37,29 @@ def energy(molecule, basis_name, method="hf", options=None, return_aux_data=Fals
 
 
 def geom_deriv(molecule, basis_name, method="hf", deriv_order=1, partial=None, options=None):
+    """Gradient (deriv_order 1) or Hessian (deriv_order 2, rounded like
+    core.py:89-91), the Hessian one column per nuclear coordinate as
+    forward-over-reverse Hessian-vector products (rhf_hessian_vector_product)."""
     if method not in ("hf", "scf", "rhf"):
         raise NotImplementedError(f"method '{method}' is outside the ERI path")
-    if deriv_order != 1:
-        raise NotImplementedError("only first derivatives are built")
+    if deriv_order not in (1, 2):
+        raise NotImplementedError("first and second derivatives are built")
+    if partial is not None and len(partial) != deriv_order:
+        raise Exception("The length of the index coordinates given by 'partial' argument should be the same "
+                        "as the order of differentiation")
     opts, eng, nelec = _setup(molecule, basis_name, options)
     with eng:
         E, aux = restricted_hartree_fock(eng, nelec, opts, return_aux_data=True)
-        g = rhf_gradient(eng, aux, precision=opts["precision"], fp32_threshold=opts["fp32_threshold"])
+        if deriv_order == 1:
+            g = rhf_gradient(eng, aux, precision=opts["precision"], fp32_threshold=opts["fp32_threshold"])
+            return np.round(g[partial[0]], 14) if partial is not None else np.round(g, 10)
+        m = 3 * molecule.natom()
+        cols = [partial[1]] if partial is not None else range(m)
+        hess = np.zeros((m, m))
+        for k in cols:
+            e = np.zeros(m)
+            e[k] = 1.0
+            hess[:, k] = rhf_hessian_vector_product(eng, aux, e)
     if partial is not None:
-        if len(partial) != deriv_order:
-            raise Exception("The length of the index coordinates given by 'partial' argument should be the same "
-                            "as the order of differentiation")
-        return np.round(g[partial[0]], 14)
-    return np.round(g, 10)
+        return np.round(hess[partial[0], partial[1]], 14)
+    return np.round(hess, 10)

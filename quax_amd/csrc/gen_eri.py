@@ -340,6 +340,54 @@ def deriv_parts(l, budget=None):
     return [part((a, a + 1), (b, b + 1), r) for a in range(NA) for b in range(NB) for r in _ranges(NC, kc)]
 
 
+# Largest SSA node count of one tangent (forward-mode) derivative part.  Dual
+# values double the live set: a (dd|dd) part over one (a, b) pair and all c
+# (3727 nodes) needs 14.3 KB of scratch per lane, which the runtime refused
+# to provide (HSA_STATUS_ERROR_OUT_OF_RESOURCES at dispatch); one c component
+# per part (1860 nodes) needs 7.2 KB, no more than the plain derivative
+# kernels that run.  Parts above this cap are split over single c components.
+TANGENT_NODES = 2000
+
+
+def tangent_parts(l):
+    """Derivative parts for the tangent kernels: the class's derivative parts,
+    each still above TANGENT_NODES split into one part per c component."""
+    out = []
+    for part in deriv_parts(l):
+        h, v, _, _, _ = build_dag(deriv_targets(l, part))
+        c0, c1 = part.get("c", (0, ncart(l[2])))
+        if len(h.lines) + len(v.lines) <= TANGENT_NODES or c1 - c0 == 1:
+            out.append(part)
+        else:
+            out += [dict(part, c=(c, c + 1)) for c in range(c0, c1)]
+    return out
+
+
+def emit_setup_tangent(L, Lt):
+    """Set-up of a tangent body: every geometric quantity is a qx_dual whose
+    derivative part follows the atoms along A_.e.tv (exponents are fixed)."""
+    L.append("  const double p = A_.pp.p[u1], q = A_.pp.p[u2];")
+    L.append("  const QxTanPair tb_ = qx_tan_pair(A_, P1, u1), tk_ = qx_tan_pair(A_, P2, u2);")
+    L.append("  const qx_dual Kab = tb_.K, Kcd = tk_.K;")
+    L.append("  const qx_dual Px = tb_.P[0], Py = tb_.P[1], Pz = tb_.P[2];")
+    L.append("  const qx_dual Qx = tk_.P[0], Qy = tk_.P[1], Qz = tk_.P[2];")
+    L.append("  const qx_dual Ax = tb_.A[0], Ay = tb_.A[1], Az = tb_.A[2];")
+    L.append("  const qx_dual Cx = tk_.A[0], Cy = tk_.A[1], Cz = tk_.A[2];")
+    L.append("  const R ABx = tb_.AB[0], ABy = tb_.AB[1], ABz = tb_.AB[2];")
+    L.append("  const R CDx = tk_.AB[0], CDy = tk_.AB[1], CDz = tk_.AB[2];")
+    L.append("  const double pq = p + q, opq = 1.0 / pq;")
+    L.append("  const qx_dual Wx = (p * Px + q * Qx) * opq, Wy = (p * Py + q * Qy) * opq, "
+             "Wz = (p * Pz + q * Qz) * opq;")
+    L.append("  const qx_dual PQx = Px - Qx, PQy = Py - Qy, PQz = Pz - Qz;")
+    L.append("  const double rho = p * q * opq;")
+    L.append("  const qx_dual T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);")
+    L.append(f"  qx_dual F[{Lt + 1}];")
+    L.append(f"  qx_boys_tan<{Lt}>(T, A_.boys, F);")
+    L.append("  const qx_dual pref = (QX_TWO_PI_5_2 / (p * q * sqrt(pq))) * (Kab * Kcd);")
+    for m in range(Lt + 1):
+        L.append(f"  const R b{m} = pref * F[{m}];")
+
+
 def emit_setup(L, Lt, deriv_w):
     L.append("  const double p = A_.pp.p[u1], Kab = A_.pp.K[u1];")
     L.append("  const double Px = A_.pp.Px[u1], Py = A_.pp.Py[u1], Pz = A_.pp.Pz[u1];")
@@ -384,10 +432,15 @@ def emit_vrr_consts(L, vrr_lines):
         L.append("  const R rq = (R)(p * opq);")
 
 
-def _head(L, fn, sig, Lt, v, vn, ret="void", inline="__forceinline__"):
-    L.append(f"template <typename R>\n__device__ {inline} {ret} {fn}(const QxArgs& A_, int P1, int P2, "
-             f"int u1, int u2{sig}) {{")
-    emit_setup(L, Lt, False)
+def _head(L, fn, sig, Lt, v, vn, ret="void", inline="__forceinline__", tangent=False):
+    if tangent:
+        L.append(f"__device__ {inline} {ret} {fn}(const QxArgs& A_, int P1, int P2, int u1, int u2{sig}) {{")
+        L.append("  using R = qx_dual;")
+        emit_setup_tangent(L, Lt)
+    else:
+        L.append(f"template <typename R>\n__device__ {inline} {ret} {fn}(const QxArgs& A_, int P1, int P2, "
+                 f"int u1, int u2{sig}) {{")
+        emit_setup(L, Lt, False)
     emit_vrr_consts(L, v.lines)
     L += ["  " + x for x in v.lines]
     for (a, c), name in sorted(vn.items()):
@@ -510,6 +563,15 @@ def emit_deriv_core(l, pi, part):
               for (e, le, last, exprs) in elems]
     L += ["  " + x if not x.startswith("  ") else x for x in interleave(h.lines, groups)]
     L.append("}")
+    # tangent array form: the derivative part of each dual value, i.e. the
+    # second derivative along A_.e.tv
+    Tn = []
+    _head(Tn, f"qx_core_t{nm}_{pi}", f", double (&dv)[{NC * NEL}]", Lt, v, vn, tangent=True)
+    Tn += wdecl
+    groups = [(last, [f"  dv[{ci * NEL + le}] = qx_tan({ex});" for ci, ex in enumerate(exprs)])
+              for (e, le, last, exprs) in elems]
+    Tn += ["  " + x if not x.startswith("  ") else x for x in interleave(h.lines, groups)]
+    Tn.append("}")
     # streaming adjoint form
     G = []
     _head(G, f"qx_gr_d{nm}_{pi}", f", double (&g)[{NC}]", Lt, v, vn)
@@ -534,7 +596,7 @@ def emit_deriv_core(l, pi, part):
     stats = dict(part=pi, a=part["a"], b=part["b"], c=(c0, c1), dinv=dinv, vrr=len(v.lines), vrr_flops=v.nflop,
                  hrr=len(h.lines), hrr_flops=h.nflop, L=Lt, N=N, nel=nel, ncomb=NC,
                  flops=core_flops(Lt, v.nflop, h.nflop) + nel * (8 + 5 * NC))
-    return "\n".join(L), "\n".join(G), stats, (cmask, xmask, dinv, NC, a0, a1, b0, b1, c0, c1)
+    return "\n".join(L), "\n".join(G), "\n".join(Tn), stats, (cmask, xmask, dinv, NC, a0, a1, b0, b1, c0, c1)
 
 
 def core_flops(L, vrr_flops, hrr_flops):
@@ -573,7 +635,8 @@ def part_groups(nparts):
 def emit_class_files(cls, outdir, fp32):
     """Writes, per class: cls_<l>_e.hip (array-form ERI kernel),
     cls_<l>_j<k>.hip (streaming J/K kernel per range of a), per derivative part
-    cls_<l>_d<k>.hip (array form) and, per group of parts, cls_<l>_g<k>.hip
+    cls_<l>_d<k>.hip (array form) and cls_<l>_t<k>.hip (tangent array form,
+    second derivatives) and, per group of parts, cls_<l>_g<k>.hip
     (streaming adjoint kernel, blockIdx.y = part within the group)."""
     la, lb, lc, ld = cls
     nm = "".join(map(str, cls))
@@ -597,13 +660,19 @@ def emit_class_files(cls, outdir, fp32):
     parts = deriv_parts(cls)
     gsrc = []
     for pi, part in enumerate(parts):
-        src_d, src_g, st, (cmask, xmask, dinv, NC, a0, a1, b0, b1, c0, c1) = emit_deriv_core(cls, pi, part)
+        src_d, src_g, _, st, (cmask, xmask, dinv, NC, a0, a1, b0, b1, c0, c1) = emit_deriv_core(cls, pi, part)
         assert (cmask, xmask, dinv, NC) == (7, 7, True, 9)
         args = f"{N}, {dims}, {cmask}, {xmask}, {1 if dinv else 0}, {NC}"
         write(f"cls_{nm}_d{pi}.hip", [src_d, "", f"QX_DEFINE_DERI_KERNEL(d{nm}_{pi}, {args}, {a0}, {a1}, {b0}, {b1}, "
                                                   f"{c0}, {c1})"])
         gsrc.append(src_g)
         stats.append((cls, f"deriv{pi}", st))
+    tparts = tangent_parts(cls)
+    for pi, part in enumerate(tparts):
+        _, _, src_t, _, (cmask, xmask, dinv, NC, a0, a1, b0, b1, c0, c1) = emit_deriv_core(cls, pi, part)
+        args = f"{N}, {dims}, {cmask}, {xmask}, {1 if dinv else 0}, {NC}"
+        write(f"cls_{nm}_t{pi}.hip", [src_t, "", f"QX_DEFINE_DERT_KERNEL(t{nm}_{pi}, {args}, {a0}, {a1}, {b0}, {b1}, "
+                                                  f"{c0}, {c1})"])
     groups = part_groups(len(parts))
     for gi, members in enumerate(groups):
         (pi,) = members
@@ -616,7 +685,7 @@ def emit_class_files(cls, outdir, fp32):
                 f"    qx_gr_d{nm}_{pi}<R>(A_, P1, P2, u1, u2, g);", "  });", "}",
                 f"QX_DEFINE_GRAD_LAUNCHER(d{nm}_g{gi}, {len(members)}, {1 if fp32 else 0})"]
         write(f"cls_{nm}_g{gi}.hip", [gsrc[pi] for pi in members] + [""] + kern)
-    return files, stats, len(parts), groups
+    return files, stats, len(parts), groups, len(tparts)
 
 
 def main():
@@ -625,9 +694,10 @@ def main():
     fp32 = "--no-fp32" not in sys.argv
     os.makedirs(outdir, exist_ok=True)
     _, classes = canonical_classes(lmax)
-    files, allstats, nparts, flops, groups, jkf = [], [], {}, {}, {}, {}
+    files, allstats, nparts, flops, groups, jkf, ntparts = [], [], {}, {}, {}, {}, {}
     for cls in classes:
-        f, s, np_, grp = emit_class_files(cls, outdir, fp32)
+        f, s, np_, grp, ntp = emit_class_files(cls, outdir, fp32)
+        ntparts[cls] = ntp
         files += f
         allstats += s
         nparts[cls] = np_
@@ -639,7 +709,8 @@ def main():
          f"#define QX_MAX_DPARTS {max(nparts.values())}",
          f"#define QX_MAX_JKPARTS {max(len(v) for v in jkf.values())}",
          "static_assert(QX_MAX_JKPARTS <= QX_JKPARTS_CAP, \"J/K part table too small\");",
-         "static_assert(QX_MAX_DPARTS <= QX_DPARTS_CAP, \"derivative part table too small\");", ""]
+         "static_assert(QX_MAX_DPARTS <= QX_DPARTS_CAP, \"derivative part table too small\");",
+         f"static_assert({max(ntparts.values())} <= QX_TPARTS_CAP, \"tangent part table too small\");", ""]
     for cls in classes:
         nm = "".join(map(str, cls))
         D.append(f'extern "C" int qx_launch_e{nm}(const QxArgs*, const QxLaunch*);')
@@ -647,6 +718,8 @@ def main():
             D.append(f'extern "C" int qx_launchj_e{nm}_{pi}(const QxArgs*, const QxLaunch*);')
         for pi in range(nparts[cls]):
             D.append(f'extern "C" int qx_launch_d{nm}_{pi}(const QxArgs*, const QxLaunch*);')
+        for pi in range(ntparts[cls]):
+            D.append(f'extern "C" int qx_launcht_t{nm}_{pi}(const QxArgs*, const QxLaunch*);')
         for gi in range(len(groups[cls])):
             D.append(f'extern "C" int qx_launchg_d{nm}_g{gi}(const QxArgs*, const QxLaunch*);')
     D.append("")
@@ -654,6 +727,7 @@ def main():
     for cls in classes:
         nm = "".join(map(str, cls))
         dl = ", ".join(f"qx_launch_d{nm}_{pi}" for pi in range(nparts[cls]))
+        tl = ", ".join(f"qx_launcht_t{nm}_{pi}" for pi in range(ntparts[cls]))
         gl = ", ".join(f"qx_launchg_d{nm}_g{gi}" for gi in range(len(groups[cls])))
         ef, pf, pc = flops[cls]
         pfs = ", ".join(f"{x:.1f}" for x in pf)
@@ -663,7 +737,7 @@ def main():
         jl = ", ".join(f"qx_launchj_e{nm}_{pi}" for pi in range(len(jkf[cls])))
         jfs = ", ".join(f"{x:.1f}" for x in jkf[cls])
         D.append(f"  {{{cls[0]}, {cls[1]}, {cls[2]}, {cls[3]}, qx_launch_e{nm}, {len(jkf[cls])}, {{{jl}}}, {{{jfs}}}, "
-                 f"{nparts[cls]}, {{{dl}}}, {len(groups[cls])}, {{{gl}}}, {ef:.1f}, {{{pfs}}}, {{{pcs}}}, {{{gfs}}}, "
+                 f"{nparts[cls]}, {{{dl}}}, {ntparts[cls]}, {{{tl}}}, {len(groups[cls])}, {{{gl}}}, {ef:.1f}, {{{pfs}}}, {{{pcs}}}, {{{gfs}}}, "
                  f"{{{gps}}}}},")
     D.append("};")
     with open(os.path.join(outdir, "qx_classes.inc"), "w") as fh:

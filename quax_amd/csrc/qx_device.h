@@ -38,7 +38,8 @@ enum QxEpi {
   QX_EPI_SCHWARZ = 2,  // diagonal (f1 f2|f1 f2) of pair P into out[P * stride + f1*nb + f2]
   QX_EPI_JK = 3,       // direct J', K' accumulation with the density
   QX_EPI_DFULL = 4,    // derivative tensor scatter (all 3N slices, or one)
-  QX_EPI_GRAD = 5      // adjoint gradient: sum dERI * density pair products
+  QX_EPI_GRAD = 5,     // adjoint gradient: sum dERI * density pair products
+  QX_EPI_JKD = 6       // J', K' of the directional derivative sum_y tv_y d(ab|cd)/dR_y
 };
 
 struct QxPairsDev {        // screened shell pairs in sorted order (SoA)
@@ -79,6 +80,9 @@ struct QxEpiDev {
   double* J; double* K;      // unsymmetrised J', K' (JK)
   int krows;                 // JK: stage K' rows of a block-uniform bra pair in LDS
   double* grad;              // 3N accumulators (GRAD)
+  const double* tv;          // 3N geometry direction: tangent kernels' seed, JKD weights
+  const int* slot_map;       // DFULL: output slice of coordinate y (-1: none); null = coord rule
+  int skip_single;           // FULL: leave out quartets with all four centres on one atom
 };
 
 struct QxArgs {
@@ -124,6 +128,71 @@ __device__ __forceinline__ void qx_boys(double T, const double* __restrict__ tab
       for (int m = 0; m < M; ++m) F[m + 1] = ((2 * m + 1) * F[m] - e) * o2t;
     }
   }
+}
+
+// ---- forward-mode (tangent) arithmetic --------------------------------
+// A value and its derivative along one geometry direction tv (3N).  Tangent
+// kernels run the generated recurrences in this type: the derivative part of
+// a first-derivative integral is then a second derivative, contracted with
+// tv on one index.
+struct qx_dual {
+  double v, d;
+  __device__ __forceinline__ qx_dual() = default;
+  __device__ __forceinline__ constexpr qx_dual(double x) : v(x), d(0.0) {}
+  __device__ __forceinline__ constexpr qx_dual(double x, double y) : v(x), d(y) {}
+};
+__device__ __forceinline__ qx_dual operator+(qx_dual a, qx_dual b) { return {a.v + b.v, a.d + b.d}; }
+__device__ __forceinline__ qx_dual operator-(qx_dual a, qx_dual b) { return {a.v - b.v, a.d - b.d}; }
+__device__ __forceinline__ qx_dual operator-(qx_dual a) { return {-a.v, -a.d}; }
+__device__ __forceinline__ qx_dual operator*(qx_dual a, qx_dual b) { return {a.v * b.v, a.v * b.d + a.d * b.v}; }
+__device__ __forceinline__ qx_dual operator*(double a, qx_dual b) { return {a * b.v, a * b.d}; }
+__device__ __forceinline__ qx_dual operator*(qx_dual a, double b) { return {a.v * b, a.d * b}; }
+__device__ __forceinline__ qx_dual operator+(double a, qx_dual b) { return {a + b.v, b.d}; }
+__device__ __forceinline__ qx_dual operator+(qx_dual a, double b) { return {a.v + b, a.d}; }
+__device__ __forceinline__ qx_dual operator-(qx_dual a, double b) { return {a.v - b, a.d}; }
+__device__ __forceinline__ qx_dual operator-(double a, qx_dual b) { return {a - b.v, -b.d}; }
+__device__ __forceinline__ qx_dual operator/(qx_dual a, double b) { return {a.v / b, a.d / b}; }
+__device__ __forceinline__ qx_dual& operator+=(qx_dual& a, qx_dual b) { a = a + b; return a; }
+__device__ __forceinline__ qx_dual& operator-=(qx_dual& a, qx_dual b) { a = a - b; return a; }
+__device__ __forceinline__ qx_dual& operator*=(qx_dual& a, qx_dual b) { a = a * b; return a; }
+// the derivative part (what a tangent kernel writes out)
+__device__ __forceinline__ double qx_tan(qx_dual x) { return x.d; }
+
+// F_m(T) along the tangent: dF_m/dT = -F_{m+1}
+template <int M>
+__device__ __forceinline__ void qx_boys_tan(qx_dual T, const double* __restrict__ tab, qx_dual (&F)[M + 1]) {
+  double f[M + 2];
+  qx_boys<M + 1>(T.v, tab, f);
+#pragma unroll
+  for (int m = 0; m <= M; ++m) F[m] = qx_dual(f[m], -T.d * f[m + 1]);
+}
+
+// Pair data of pseudo pair u (shell pair P) with its derivative along tv:
+// centres move with their atoms, P = (alpha A + beta B) / p,
+// K = c_a c_b exp(-alpha beta / p |A - B|^2).
+struct QxTanPair {
+  qx_dual K, P[3], A[3], AB[3];
+};
+__device__ __forceinline__ QxTanPair qx_tan_pair(const QxArgs& A_, int P, int u) {
+  const double* tv = A_.e.tv;
+  const int ta = 3 * A_.pr.atom_a[P], tb = 3 * A_.pr.atom_b[P];
+  const double p = A_.pp.p[u], al = 0.5 * A_.pp.a2[u], be = 0.5 * A_.pp.b2[u];
+  const double ab[3] = {A_.pr.ABx[P], A_.pr.ABy[P], A_.pr.ABz[P]};
+  const double pv[3] = {A_.pp.Px[u], A_.pp.Py[u], A_.pp.Pz[u]};
+  const double av[3] = {A_.pr.Ax[P], A_.pr.Ay[P], A_.pr.Az[P]};
+  QxTanPair r;
+  double abd = 0.0;
+#pragma unroll
+  for (int x = 0; x < 3; ++x) {
+    const double dA = tv[ta + x], dB = tv[tb + x];
+    r.A[x] = qx_dual(av[x], dA);
+    r.AB[x] = qx_dual(ab[x], dA - dB);
+    r.P[x] = qx_dual(pv[x], (al * dA + be * dB) / p);
+    abd += ab[x] * (dA - dB);
+  }
+  const double K = A_.pp.K[u];
+  r.K = qx_dual(K, -2.0 * al * be / p * abd * K);
+  return r;
 }
 
 __device__ __forceinline__ void qx_locate(const QxArgs& A_, long long t, int& u1, int& u2, int& P1, int& P2) {
@@ -322,6 +391,15 @@ __device__ __forceinline__ bool qx_single_atom(const QxArgs& A_, int P1, int P2)
   return a == A_.pr.atom_b[P1] && a == A_.pr.atom_a[P2] && a == A_.pr.atom_b[P2];
 }
 
+// Output slice of nuclear coordinate y in a derivative-tensor scatter:
+// slot_map[y] when given (-1 = not wanted), else every slice (coord < 0) or
+// only coordinate `coord` into slice 0.
+__device__ __forceinline__ long long qx_dfull_slice(const QxEpiDev& e, int y) {
+  if (e.slot_map) return e.slot_map[y];
+  if (e.coord >= 0) return y == e.coord ? 0 : -1;
+  return y;
+}
+
 // A part covers the output functions with a in [A0, A1), b in [B0, B1) and c
 // in [C0, C1): dv holds NCOMB blocks of that part's NEL elements, in
 // (a, b, c, d) order.
@@ -342,7 +420,6 @@ __device__ __forceinline__ void qx_epi_deri(
     const size_t nn = n;
     const size_t n4 = nn * nn * nn * nn;
     const bool sab = A_.pr.sa[P1] == A_.pr.sb[P1], scd = A_.pr.sa[P2] == A_.pr.sb[P2], sp = P1 == P2;
-    const int coord = A_.e.coord;
 #pragma unroll
     for (int i = A0; i < A1; ++i)
 #pragma unroll
@@ -362,23 +439,116 @@ __device__ __forceinline__ void qx_epi_deri(
                 if (!(XMASK & (1 << x))) continue;
                 const double v = (double)dv[(ci++) * NEL + e];
                 dsum[x] += v;
-                const int slot = 3 * at[cen] + x;
-                if (coord >= 0 && slot != coord) continue;
-                double* base = A_.e.out + (coord >= 0 ? 0 : (size_t)slot * n4);
-                qx_scatter8(base, nn, ia + i, ib + j, ic + k, id + l, v, sab, scd, sp, true);
+                const long long sl = qx_dfull_slice(A_.e, 3 * at[cen] + x);
+                if (sl < 0) continue;
+                qx_scatter8(A_.e.out + (size_t)sl * n4, nn, ia + i, ib + j, ic + k, id + l, v, sab, scd, sp, true);
               }
             }
             if (DINV) {
 #pragma unroll
               for (int x = 0; x < 3; ++x) {
-                const int slot = 3 * at[3] + x;
-                if (coord >= 0 && slot != coord) continue;
-                double* base = A_.e.out + (coord >= 0 ? 0 : (size_t)slot * n4);
-                qx_scatter8(base, nn, ia + i, ib + j, ic + k, id + l, -dsum[x], sab, scd, sp, true);
+                const long long sl = qx_dfull_slice(A_.e, 3 * at[3] + x);
+                if (sl < 0) continue;
+                qx_scatter8(A_.e.out + (size_t)sl * n4, nn, ia + i, ib + j, ic + k, id + l, -dsum[x], sab, scd, sp,
+                            true);
               }
             }
           }
   }
+}
+
+// Derivative values of one element e (local index) of a part spread to the
+// 12 centre x direction slots; d/dD from translational invariance (DINV).
+template <int CMASK, int XMASK, int DINV, int NCOMB, int NEL>
+__device__ __forceinline__ void qx_elem_spread(const double (&dv)[NCOMB * NEL], int e, double (&gv)[12]) {
+  int ci = 0;
+  double dsum[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int cen = 0; cen < 4; ++cen)
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+      double v = 0.0;
+      if ((CMASK & (1 << cen)) && (XMASK & (1 << x))) v = dv[(ci++) * NEL + e];
+      dsum[x] += v;
+      gv[3 * cen + x] = v;
+    }
+  if (DINV) {
+#pragma unroll
+    for (int x = 0; x < 3; ++x) gv[9 + x] = -dsum[x];
+  }
+}
+
+// Adjoint contraction of an array of derivative values (tangent kernels):
+// gv[12] = 8 f sum_e Gamma_e dv_e, Gamma_ijkl = 2 D_ij D_kl - (D_ik D_jl +
+// D_il D_jk) / 2 with the degeneracy weight f of the canonical quartet.
+template <int NA, int NB, int NC, int ND, int CMASK, int XMASK, int DINV, int NCOMB, int A0, int A1, int B0, int B1,
+          int C0, int C1>
+__device__ __forceinline__ void qx_epi_dgrad(
+    const QxArgs& A_, int P1, int P2,
+    const double (&dv)[NCOMB * QxPartShape<NA, NB, NC, ND, A0, A1, B0, B1, C0, C1>::NEL], double (&gv)[12]) {
+  constexpr int NEL = QxPartShape<NA, NB, NC, ND, A0, A1, B0, B1, C0, C1>::NEL;
+  const size_t n = A_.e.nbf;
+  const double* D = A_.e.D;
+  const int ia = A_.pr.bfa[P1], ib = A_.pr.bfb[P1], ic = A_.pr.bfa[P2], id = A_.pr.bfb[P2];
+#pragma unroll
+  for (int k = 0; k < 12; ++k) gv[k] = 0.0;
+  for (int i = A0; i < A1; ++i)
+    for (int j = B0; j < B1; ++j)
+      for (int k = C0; k < C1; ++k)
+        for (int l = 0; l < ND; ++l) {
+          const int e = ((((i - A0) * (B1 - B0) + (j - B0)) * (C1 - C0)) + (k - C0)) * ND + l;
+          const size_t I = ia + i, J = ib + j, K = ic + k, L = id + l;
+          const double G = 2.0 * D[I * n + J] * D[K * n + L] - 0.5 * (D[I * n + K] * D[J * n + L] +
+                                                                      D[I * n + L] * D[J * n + K]);
+          double ge[12];
+          qx_elem_spread<CMASK, XMASK, DINV, NCOMB, NEL>(dv, e, ge);
+#pragma unroll
+          for (int c = 0; c < 12; ++c) gv[c] += G * ge[c];
+        }
+  const double w = 8.0 * qx_degeneracy(A_, P1, P2);
+#pragma unroll
+  for (int k = 0; k < 12; ++k) gv[k] *= w;
+}
+
+// J', K' of the directional derivative of each integral, s = sum over the
+// 12 centre x direction slots of tv[3 atom + x] d(ab|cd), with the J/K
+// kernels' accumulation rule (J'_ab += 4 f s D_cd, K'_ac += 2 f s D_bd, ...).
+template <int NA, int NB, int NC, int ND, int CMASK, int XMASK, int DINV, int NCOMB, int A0, int A1, int B0, int B1,
+          int C0, int C1>
+__device__ __forceinline__ void qx_epi_djk(
+    const QxArgs& A_, int P1, int P2,
+    const double (&dv)[NCOMB * QxPartShape<NA, NB, NC, ND, A0, A1, B0, B1, C0, C1>::NEL]) {
+  constexpr int NEL = QxPartShape<NA, NB, NC, ND, A0, A1, B0, B1, C0, C1>::NEL;
+  const size_t n = A_.e.nbf;
+  const double* D = A_.e.D;
+  const double* tv = A_.e.tv;
+  const int at[4] = {A_.pr.atom_a[P1], A_.pr.atom_b[P1], A_.pr.atom_a[P2], A_.pr.atom_b[P2]};
+  double w[12];
+#pragma unroll
+  for (int k = 0; k < 12; ++k) w[k] = tv[3 * at[k / 3] + k % 3];
+  const int ia = A_.pr.bfa[P1], ib = A_.pr.bfb[P1], ic = A_.pr.bfa[P2], id = A_.pr.bfb[P2];
+  const double f = qx_degeneracy(A_, P1, P2), fj = 4.0 * f, fk = 2.0 * f;
+  double* J = A_.e.J;
+  double* K = A_.e.K;
+  for (int i = A0; i < A1; ++i)
+    for (int j = B0; j < B1; ++j)
+      for (int k = C0; k < C1; ++k)
+        for (int l = 0; l < ND; ++l) {
+          const int e = ((((i - A0) * (B1 - B0) + (j - B0)) * (C1 - C0)) + (k - C0)) * ND + l;
+          double ge[12];
+          qx_elem_spread<CMASK, XMASK, DINV, NCOMB, NEL>(dv, e, ge);
+          double s = 0.0;
+#pragma unroll
+          for (int c = 0; c < 12; ++c) s += w[c] * ge[c];
+          if (s == 0.0) continue;
+          const size_t I = ia + i, Jx = ib + j, Kx = ic + k, L = id + l;
+          qx_atomic_add(&J[I * n + Jx], fj * s * D[Kx * n + L]);
+          qx_atomic_add(&J[Kx * n + L], fj * s * D[I * n + Jx]);
+          qx_atomic_add(&K[I * n + Kx], fk * s * D[Jx * n + L]);
+          qx_atomic_add(&K[I * n + L], fk * s * D[Jx * n + Kx]);
+          qx_atomic_add(&K[Jx * n + Kx], fk * s * D[I * n + L]);
+          qx_atomic_add(&K[Jx * n + L], fk * s * D[I * n + Kx]);
+        }
 }
 
 // ---- kernel definitions (instantiated once per generated file) --------
@@ -393,6 +563,7 @@ __device__ __forceinline__ void qx_epi_deri(
     if (t >= A_.w.nquart) return;                                                                 \
     int u1, u2, P1, P2;                                                                           \
     qx_locate(A_, t, u1, u2, P1, P2);                                                             \
+    if (A_.e.skip_single && qx_single_atom(A_, P1, P2)) return;                                   \
     double o[N];                                                                                  \
     qx_core_##NAME<double>(A_, P1, P2, u1, u2, o);                                                \
     qx_epi_eri<NA, NB, NC, ND, double>(A_, t, P1, P2, o, true);                                   \
@@ -489,12 +660,68 @@ __device__ __forceinline__ void qx_epi_deri(
                   "per-lane derivative block must stay small (scratch is allocated per lane)");    \
     double dv[NCOMB * Shape::NEL];                                                                \
     qx_core_##NAME<double>(A_, P1, P2, u1, u2, dv);                                               \
-    qx_epi_deri<NA, NB, NC, ND, CMASK, XMASK, DINV, NCOMB, A0, A1, B0, B1, C0, C1, double>(A_, P1, P2, dv); \
+    if (A_.e.epi == QX_EPI_JKD)                                                                   \
+      qx_epi_djk<NA, NB, NC, ND, CMASK, XMASK, DINV, NCOMB, A0, A1, B0, B1, C0, C1>(A_, P1, P2, dv); \
+    else                                                                                          \
+      qx_epi_deri<NA, NB, NC, ND, CMASK, XMASK, DINV, NCOMB, A0, A1, B0, B1, C0, C1, double>(A_, P1, P2, dv); \
   }                                                                                               \
   extern "C" int qx_launch_##NAME(const QxArgs* a, const QxLaunch* L) {                           \
     if (a->w.nquart <= 0) return 0;                                                               \
-    if (L->fp32 || a->e.epi != QX_EPI_DFULL) return -1;                                           \
+    if (L->fp32 || (a->e.epi != QX_EPI_DFULL && a->e.epi != QX_EPI_JKD)) return -1;               \
     hipLaunchKernelGGL(qx_k_##NAME, QX_GRID(a->w.nquart), dim3(QX_BLOCK), 0, L->stream, *a);      \
+    return (int)hipGetLastError();                                                                \
+  }
+
+// Tangent array-form derivative kernel: the part's derivative integrals are
+// differentiated once more along tv (forward mode, qx_dual recursion), so dv
+// holds sum_x tv_x d^2(ab|cd)/dR_x dR_y per centre x direction slot y.
+// Epilogues: DFULL (second-derivative tensor slices through slot_map) or
+// GRAD (the adjoint contraction with the density pair products, per block in
+// LDS).  The GRAD path's barriers run in block-uniform control flow.
+#define QX_DEFINE_DERT_KERNEL(NAME, N, NA, NB, NC, ND, CMASK, XMASK, DINV, NCOMB, A0, A1, B0, B1, C0, C1) \
+  __global__ __launch_bounds__(QX_BLOCK) void qx_kt_##NAME(QxArgs A_) {                           \
+    __shared__ double sg[QX_GRAD_LDS];                                                            \
+    using Shape = QxPartShape<NA, NB, NC, ND, A0, A1, B0, B1, C0, C1>;                            \
+    static_assert(NCOMB * Shape::NEL * 8 <= 16384,                                                \
+                  "per-lane derivative block must stay small (scratch is allocated per lane)");    \
+    const bool grad = A_.e.epi == QX_EPI_GRAD;                                                    \
+    const int ng = 3 * A_.e.natom;                                                                \
+    const bool lds = grad && ng <= QX_GRAD_LDS;                                                   \
+    if (lds) {                                                                                    \
+      for (int i = threadIdx.x; i < ng; i += QX_BLOCK) sg[i] = 0.0;                               \
+      __syncthreads();                                                                            \
+    }                                                                                             \
+    const long long t = (long long)blockIdx.x * QX_BLOCK + threadIdx.x;                          \
+    if (t < A_.w.nquart) {                                                                        \
+      int u1, u2, P1, P2;                                                                         \
+      qx_locate(A_, t, u1, u2, P1, P2);                                                           \
+      if (!qx_single_atom(A_, P1, P2)) {                                                          \
+        double dv[NCOMB * Shape::NEL];                                                            \
+        qx_core_##NAME(A_, P1, P2, u1, u2, dv);                                                   \
+        if (grad) {                                                                               \
+          double gv[12];                                                                          \
+          qx_epi_dgrad<NA, NB, NC, ND, CMASK, XMASK, DINV, NCOMB, A0, A1, B0, B1, C0, C1>(A_, P1, P2, dv, gv); \
+          const int at[4] = {A_.pr.atom_a[P1], A_.pr.atom_b[P1], A_.pr.atom_a[P2], A_.pr.atom_b[P2]}; \
+          for (int k = 0; k < 12; ++k) {                                                          \
+            const int slot = 3 * at[k / 3] + k % 3;                                               \
+            if (lds) atomicAdd(&sg[slot], gv[k]);                                                 \
+            else qx_atomic_add(&A_.e.grad[slot], gv[k]);                                          \
+          }                                                                                       \
+        } else {                                                                                  \
+          qx_epi_deri<NA, NB, NC, ND, CMASK, XMASK, DINV, NCOMB, A0, A1, B0, B1, C0, C1, double>(A_, P1, P2, dv); \
+        }                                                                                         \
+      }                                                                                           \
+    }                                                                                             \
+    if (lds) {                                                                                    \
+      __syncthreads();                                                                            \
+      for (int i = threadIdx.x; i < ng; i += QX_BLOCK)                                            \
+        if (sg[i] != 0.0) qx_atomic_add(&A_.e.grad[i], sg[i]);                                    \
+    }                                                                                             \
+  }                                                                                               \
+  extern "C" int qx_launcht_##NAME(const QxArgs* a, const QxLaunch* L) {                          \
+    if (a->w.nquart <= 0) return 0;                                                               \
+    if (L->fp32 || (a->e.epi != QX_EPI_DFULL && a->e.epi != QX_EPI_GRAD) || !a->e.tv) return -1;  \
+    hipLaunchKernelGGL(qx_kt_##NAME, QX_GRID(a->w.nquart), dim3(QX_BLOCK), 0, L->stream, *a);     \
     return (int)hipGetLastError();                                                                \
   }
 
@@ -558,6 +785,7 @@ __device__ __forceinline__ void qx_grad_body(const QxArgs& A_, double* sg, Part 
 
 #define QX_DPARTS_CAP 128  // most derivative parts one class may be split into
 #define QX_JKPARTS_CAP 8  // most J/K parts (ranges of a components) of one class
+#define QX_TPARTS_CAP 256  // most tangent (second-derivative) parts of one class
 
 struct QxClassEntry {
   int la, lb, lc, ld;
@@ -567,6 +795,8 @@ struct QxClassEntry {
   double jk_flops[QX_JKPARTS_CAP];   // FP operations per primitive quartet, J/K epilogue included
   int nparts;
   int (*deri[QX_DPARTS_CAP])(const QxArgs*, const QxLaunch*);  // array form (derivative tensor), per part
+  int ntparts;
+  int (*dert[QX_TPARTS_CAP])(const QxArgs*, const QxLaunch*);  // tangent array form (second derivatives), per part
   int ngroups;
   int (*grad[QX_DPARTS_CAP])(const QxArgs*, const QxLaunch*);  // streaming adjoint gradient, per part group
   double eri_flops;                   // algorithmic FP operations of one ERI body (J/K epilogue excluded)

@@ -123,19 +123,42 @@ __device__ void oe_boys(int M, double T, double* F) {
   }
 }
 
-struct OePrim {
-  double a, b, p, K;   // exponents, p = a+b, K = ca cb exp(-ab/p AB^2)
-  double PA[3], PB[3], AB[3];
+// The one-electron helpers are templated on the scalar S of the geometric
+// quantities: double, or qx_dual for the tangent (forward-mode) kernel that
+// differentiates the first-derivative contraction once more along a
+// geometry direction.  Exponents stay double.
+__device__ __forceinline__ double oe_exp(double x) { return exp(x); }
+__device__ __forceinline__ qx_dual oe_exp(qx_dual x) {
+  const double e = exp(x.v);
+  return qx_dual(e, e * x.d);
+}
+__device__ __forceinline__ double oe_ipow(double x, int n) { return pow(x, n); }
+__device__ __forceinline__ qx_dual oe_ipow(qx_dual x, int n) {
+  qx_dual r(1.0);
+  for (int i = 0; i < n; ++i) r = r * x;
+  return r;
+}
+// what a contraction accumulates: the value, or the derivative part
+__device__ __forceinline__ double oe_out(double x) { return x; }
+__device__ __forceinline__ double oe_out(qx_dual x) { return x.d; }
+
+template <typename S>
+struct OePrimT {
+  double a, b, p;      // exponents, p = a+b
+  S K;                 // ca cb exp(-ab/p AB^2)
+  S PA[3], PB[3], AB[3];
 };
+using OePrim = OePrimT<double>;
 
 // 1D overlap tables s[d][i][j] (no prefactor), i <= imax, j <= jmax
-__device__ void oe_s1d(const OePrim& P, int imax, int jmax, double (*s)[OE_LMAX + 2][OE_JMAX + 2]) {
+template <typename S>
+__device__ void oe_s1d(const OePrimT<S>& P, int imax, int jmax, S (*s)[OE_LMAX + 2][OE_JMAX + 2]) {
   const double o2p = 0.5 / P.p;
   for (int d = 0; d < 3; ++d)
     for (int j = 0; j <= jmax; ++j)
       for (int i = 0; i <= imax; ++i) {
-        double v;
-        if (i == 0 && j == 0) v = 1.0;
+        S v;
+        if (i == 0 && j == 0) v = S(1.0);
         else if (j == 0) {
           v = P.PA[d] * s[d][i - 1][0];
           if (i > 1) v += o2p * (i - 1) * s[d][i - 2][0];
@@ -149,29 +172,38 @@ __device__ void oe_s1d(const OePrim& P, int imax, int jmax, double (*s)[OE_LMAX 
 }
 
 // S and T for component pair (a, b) from the 1D tables
-__device__ __forceinline__ void oe_st(const OePrim& P, const int* a, const int* b,
-                                      double (*s)[OE_LMAX + 2][OE_JMAX + 2], double& S, double& T) {
+template <typename S>
+__device__ __forceinline__ void oe_st(const OePrimT<S>& P, const int* a, const int* b,
+                                      S (*s)[OE_LMAX + 2][OE_JMAX + 2], S& Sv, S& T) {
   const double pref = pow(3.14159265358979323846 / P.p, 1.5);
-  double sv[3], tv[3];
+  S sv[3], tv[3];
   for (int d = 0; d < 3; ++d) {
     const int i = a[d], j = b[d];
     sv[d] = s[d][i][j];
-    double t = P.b * (2 * j + 1) * s[d][i][j] - 2.0 * P.b * P.b * s[d][i][j + 2];
-    if (j > 1) t -= 0.5 * j * (j - 1) * s[d][i][j - 2];
+    S t = P.b * (2 * j + 1) * s[d][i][j] - 2.0 * P.b * P.b * s[d][i][j + 2];
+    if (j > 1) t = t - 0.5 * j * (j - 1) * s[d][i][j - 2];
     tv[d] = t;
   }
-  S = pref * sv[0] * sv[1] * sv[2];
+  Sv = pref * sv[0] * sv[1] * sv[2];
   T = pref * (tv[0] * sv[1] * sv[2] + sv[0] * tv[1] * sv[2] + sv[0] * sv[1] * tv[2]);
 }
 
+// Boys function of a dual argument: dF_m/dT = -F_{m+1}
+__device__ void oe_boys(int M, qx_dual T, qx_dual* F) {
+  double f[OE_LT + 3];
+  oe_boys(M + 1, T.v, f);
+  for (int m = 0; m <= M; ++m) F[m] = qx_dual(f[m], -T.d * f[m + 1]);
+}
+
 // [e|0]^(0) for |e| <= L toward a unit charge at C (no -Z, includes 2pi/p K)
-__device__ void oe_vrr(const OePrim& P, const double* Pc, const double* C, int L, double* out /*coff(L+1)*/) {
-  double V[OE_LT + 1][56];
-  double PC[3] = {Pc[0] - C[0], Pc[1] - C[1], Pc[2] - C[2]};
-  const double T = P.p * (PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2]);
-  double F[OE_LT + 2];
+template <typename S>
+__device__ void oe_vrr(const OePrimT<S>& P, const S* Pc, const S* C, int L, S* out /*coff(L+1)*/) {
+  S V[OE_LT + 1][56];
+  S PC[3] = {Pc[0] - C[0], Pc[1] - C[1], Pc[2] - C[2]};
+  const S T = P.p * (PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2]);
+  S F[OE_LT + 2];
   oe_boys(L, T, F);
-  const double pref = 2.0 * 3.14159265358979323846 / P.p * P.K;
+  const S pref = 2.0 * 3.14159265358979323846 / P.p * P.K;
   for (int m = 0; m <= L; ++m) V[m][0] = pref * F[m];
   const double o2p = 0.5 / P.p;
   for (int l = 1; l <= L; ++l)
@@ -189,8 +221,8 @@ __device__ void oe_vrr(const OePrim& P, const double* Pc, const double* C, int L
           g2 = d_gidx(e2[0], e2[1], e2[2]);
         }
         for (int m = 0; m <= L - l; ++m) {
-          double v = P.PA[ax] * V[m][g1] - PC[ax] * V[m + 1][g1];
-          if (g2 >= 0) v += o2p * e1[ax] * (V[m][g2] - V[m + 1][g2]);
+          S v = P.PA[ax] * V[m][g1] - PC[ax] * V[m + 1][g1];
+          if (g2 >= 0) v = v + o2p * e1[ax] * (V[m][g2] - V[m + 1][g2]);
           V[m][g] = v;
         }
       }
@@ -204,14 +236,15 @@ __device__ __forceinline__ double binom(int n, int k) {
 }
 
 // (a|V_C|b) from the [e|0] table by the binomial transfer
-__device__ double oe_vab(const double* E, const int* a, const int* b, const double* AB) {
-  double acc = 0.0;
+template <typename S>
+__device__ S oe_vab(const S* E, const int* a, const int* b, const S* AB) {
+  S acc(0.0);
   for (int kx = 0; kx <= b[0]; ++kx)
     for (int ky = 0; ky <= b[1]; ++ky)
       for (int kz = 0; kz <= b[2]; ++kz) {
-        const double c = binom(b[0], kx) * binom(b[1], ky) * binom(b[2], kz) * pow(AB[0], b[0] - kx) *
-                         pow(AB[1], b[1] - ky) * pow(AB[2], b[2] - kz);
-        acc += c * E[d_gidx(a[0] + kx, a[1] + ky, a[2] + kz)];
+        const S c = binom(b[0], kx) * binom(b[1], ky) * binom(b[2], kz) * oe_ipow(AB[0], b[0] - kx) *
+                    oe_ipow(AB[1], b[1] - ky) * oe_ipow(AB[2], b[2] - kz);
+        acc = acc + c * E[d_gidx(a[0] + kx, a[1] + ky, a[2] + kz)];
       }
   return acc;
 }
@@ -222,22 +255,36 @@ __device__ void unpack(int code, int* c) {
   c[2] = code & 0xff;
 }
 
-__device__ OePrim oe_prim(const QxShellsDev& sh, int sa, int sb, int ia, int ib) {
-  OePrim P;
-  const double* A = sh.xyz + 3 * sh.atom[sa];
-  const double* B = sh.xyz + 3 * sh.atom[sb];
+// Centre coordinate of an atom: its position, moving along tv for duals.
+template <typename S>
+__device__ __forceinline__ void oe_centre(const QxShellsDev& sh, const double* tv, int atom, S* X);
+template <>
+__device__ __forceinline__ void oe_centre<double>(const QxShellsDev& sh, const double*, int atom, double* X) {
+  for (int d = 0; d < 3; ++d) X[d] = sh.xyz[3 * atom + d];
+}
+template <>
+__device__ __forceinline__ void oe_centre<qx_dual>(const QxShellsDev& sh, const double* tv, int atom, qx_dual* X) {
+  for (int d = 0; d < 3; ++d) X[d] = qx_dual(sh.xyz[3 * atom + d], tv[3 * atom + d]);
+}
+
+template <typename S>
+__device__ OePrimT<S> oe_prim(const QxShellsDev& sh, int sa, int sb, int ia, int ib, const double* tv = nullptr) {
+  OePrimT<S> P;
+  S A[3], B[3];
+  oe_centre<S>(sh, tv, sh.atom[sa], A);
+  oe_centre<S>(sh, tv, sh.atom[sb], B);
   P.a = sh.exps[ia];
   P.b = sh.exps[ib];
   P.p = P.a + P.b;
-  double AB2 = 0.0;
+  S AB2(0.0);
   for (int d = 0; d < 3; ++d) {
-    const double Pd = (P.a * A[d] + P.b * B[d]) / P.p;
+    const S Pd = (P.a * A[d] + P.b * B[d]) / P.p;
     P.PA[d] = Pd - A[d];
     P.PB[d] = Pd - B[d];
     P.AB[d] = A[d] - B[d];
-    AB2 += P.AB[d] * P.AB[d];
+    AB2 = AB2 + P.AB[d] * P.AB[d];
   }
-  P.K = sh.coefs[ia] * sh.coefs[ib] * exp(-P.a * P.b / P.p * AB2);
+  P.K = sh.coefs[ia] * sh.coefs[ib] * oe_exp(-P.a * P.b / P.p * AB2);
   return P;
 }
 
@@ -255,7 +302,7 @@ __global__ void k_oei(QxShellsDev sh, double* S, double* T, double* V) {
   double E[84];
   for (int ia = sh.poff[sa]; ia < sh.poff[sa] + sh.nprim[sa]; ++ia)
     for (int ib = sh.poff[sb]; ib < sh.poff[sb] + sh.nprim[sb]; ++ib) {
-      const OePrim P = oe_prim(sh, sa, sb, ia, ib);
+      const OePrim P = oe_prim<double>(sh, sa, sb, ia, ib);
       oe_s1d(P, la, lb + 2, s1);
       for (int i = 0; i < na; ++i)
         for (int j = 0; j < nb; ++j) {
@@ -298,8 +345,12 @@ __global__ void k_oei(QxShellsDev sh, double* S, double* T, double* V) {
 // takes the terms: 0 the overlap and kinetic ones, 1 + C the attraction of
 // nucleus C (rows stride by gridDim.y), so a pair's nucleus loop is spread
 // over the grid instead of run by one lane.
+// S = qx_dual (tangent mode, tv = geometry direction, tensors not written):
+// every atom moves along tv and the contraction accumulates the derivative
+// part, grad += Pd.d/dtv d(T+V) + Wd.d/dtv dS.
+template <typename S>
 __global__ void k_oei_deriv(QxShellsDev sh, const double* Pd, const double* Wd, double* grad, double* dS, double* dT,
-                            double* dV) {
+                            double* dV, const double* tv) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= sh.nshell * sh.nshell) return;
   const int sa = t / sh.nshell, sb = t % sh.nshell;
@@ -308,13 +359,13 @@ __global__ void k_oei_deriv(QxShellsDev sh, const double* Pd, const double* Wd, 
   const int n = sh.ncart;
   const size_t nn = (size_t)n * n;
   const int atA = sh.atom[sa], atB = sh.atom[sb];
-  double s1[3][OE_LMAX + 2][OE_JMAX + 2];
-  double E[84];
+  S s1[3][OE_LMAX + 2][OE_JMAX + 2];
+  S E[84];
   for (int term = blockIdx.y; term <= sh.natom; term += gridDim.y) {
     double gAB[2][3] = {{0, 0, 0}, {0, 0, 0}};
     for (int ia = sh.poff[sa]; ia < sh.poff[sa] + sh.nprim[sa]; ++ia)
       for (int ib = sh.poff[sb]; ib < sh.poff[sb] + sh.nprim[sb]; ++ib) {
-        const OePrim P = oe_prim(sh, sa, sb, ia, ib);
+        const OePrimT<S> P = oe_prim<S>(sh, sa, sb, ia, ib, tv);
         if (term == 0) {  // overlap and kinetic
           oe_s1d(P, la + 1, lb + 3, s1);
           for (int i = 0; i < na; ++i)
@@ -329,14 +380,14 @@ __global__ void k_oei_deriv(QxShellsDev sh, const double* Pd, const double* Wd, 
                 for (int x = 0; x < 3; ++x) {
                   int au[3] = {a[0], a[1], a[2]}, bu[3] = {b[0], b[1], b[2]};
                   if (side == 0) au[x] += 1; else bu[x] += 1;
-                  double su, tu;
+                  S su, tu;
                   oe_st(P, au, bu, s1, su, tu);
-                  double ds = ex2 * su, dt = ex2 * tu;
+                  S ds = ex2 * su, dt = ex2 * tu;
                   const int q = side == 0 ? a[x] : b[x];
                   if (q > 0) {
                     int ad[3] = {a[0], a[1], a[2]}, bd[3] = {b[0], b[1], b[2]};
                     if (side == 0) ad[x] -= 1; else bd[x] -= 1;
-                    double sd, td;
+                    S sd, td;
                     oe_st(P, ad, bd, s1, sd, td);
                     ds -= q * sd;
                     dt -= q * td;
@@ -344,10 +395,10 @@ __global__ void k_oei_deriv(QxShellsDev sh, const double* Pd, const double* Wd, 
                   ds *= P.K;
                   dt *= P.K;
                   const int at = side == 0 ? atA : atB;
-                  if (grad) gAB[side][x] += pd * dt + wd * ds;
+                  if (grad) gAB[side][x] += pd * oe_out(dt) + wd * oe_out(ds);
                   if (dS) {
-                    atomicAdd(&dS[(size_t)(3 * at + x) * nn + rc], ds);
-                    atomicAdd(&dT[(size_t)(3 * at + x) * nn + rc], dt);
+                    atomicAdd(&dS[(size_t)(3 * at + x) * nn + rc], oe_out(ds));
+                    atomicAdd(&dT[(size_t)(3 * at + x) * nn + rc], oe_out(dt));
                   }
                 }
               }
@@ -356,10 +407,11 @@ __global__ void k_oei_deriv(QxShellsDev sh, const double* Pd, const double* Wd, 
         }
         // attraction of nucleus atC
         const int atC = term - 1;
-        double Pc[3];
-        const double* A = sh.xyz + 3 * sh.atom[sa];
-        for (int d = 0; d < 3; ++d) Pc[d] = A[d] + P.PA[d];
-        oe_vrr(P, Pc, sh.xyz + 3 * atC, la + lb + 1, E);
+        S Pc[3], Ac[3], Cc[3];
+        oe_centre<S>(sh, tv, sh.atom[sa], Ac);
+        oe_centre<S>(sh, tv, atC, Cc);
+        for (int d = 0; d < 3; ++d) Pc[d] = Ac[d] + P.PA[d];
+        oe_vrr(P, Pc, Cc, la + lb + 1, E);
         double gC[3] = {0, 0, 0};
         for (int i = 0; i < na; ++i)
           for (int j = 0; j < nb; ++j) {
@@ -373,22 +425,23 @@ __global__ void k_oei_deriv(QxShellsDev sh, const double* Pd, const double* Wd, 
               for (int x = 0; x < 3; ++x) {
                 int au[3] = {a[0], a[1], a[2]}, bu[3] = {b[0], b[1], b[2]};
                 if (side == 0) au[x] += 1; else bu[x] += 1;
-                double dv = ex2 * oe_vab(E, au, bu, P.AB);
+                S dv = ex2 * oe_vab(E, au, bu, P.AB);
                 const int q = side == 0 ? a[x] : b[x];
                 if (q > 0) {
                   int ad[3] = {a[0], a[1], a[2]}, bd[3] = {b[0], b[1], b[2]};
                   if (side == 0) ad[x] -= 1; else bd[x] -= 1;
                   dv -= q * oe_vab(E, ad, bd, P.AB);
                 }
-                dv *= -sh.Z[atC];
+                dv *= S(-sh.Z[atC]);
                 const int at = side == 0 ? atA : atB;
+                const double dvo = oe_out(dv);
                 if (grad) {
-                  gAB[side][x] += pd * dv;
-                  gC[x] -= pd * dv;
+                  gAB[side][x] += pd * dvo;
+                  gC[x] -= pd * dvo;
                 }
                 if (dV) {
-                  atomicAdd(&dV[(size_t)(3 * at + x) * nn + rc], dv);
-                  atomicAdd(&dV[(size_t)(3 * atC + x) * nn + rc], -dv);
+                  atomicAdd(&dV[(size_t)(3 * at + x) * nn + rc], dvo);
+                  atomicAdd(&dV[(size_t)(3 * atC + x) * nn + rc], -dvo);
                 }
               }
             }
@@ -450,8 +503,16 @@ int qxk_oei(const QxShellsDev* sh, double* S, double* T, double* V, hipStream_t 
 int qxk_oei_deriv(const QxShellsDev* sh, const double* Pd, const double* Wd, double* grad, double* dS, double* dT,
                   double* dV, hipStream_t s) {
   const unsigned rows = (unsigned)std::min(sh->natom + 1, 65535);
-  hipLaunchKernelGGL(k_oei_deriv, dim3(nblk((long long)sh->nshell * sh->nshell, 64), rows), dim3(64), 0, s, *sh, Pd, Wd,
-                     grad, dS, dT, dV);
+  hipLaunchKernelGGL(k_oei_deriv<double>, dim3(nblk((long long)sh->nshell * sh->nshell, 64), rows), dim3(64), 0, s, *sh,
+                     Pd, Wd, grad, dS, dT, dV, (const double*)nullptr);
+  return (int)hipGetLastError();
+}
+
+int qxk_oei_hess_dir(const QxShellsDev* sh, const double* Pd, const double* Wd, const double* tv, double* grad,
+                     hipStream_t s) {
+  const unsigned rows = (unsigned)std::min(sh->natom + 1, 65535);
+  hipLaunchKernelGGL(k_oei_deriv<qx_dual>, dim3(nblk((long long)sh->nshell * sh->nshell, 64), rows), dim3(64), 0, s,
+                     *sh, Pd, Wd, grad, (double*)nullptr, (double*)nullptr, (double*)nullptr, tv);
   return (int)hipGetLastError();
 }
 

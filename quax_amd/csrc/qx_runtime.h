@@ -139,4 +139,7 @@ int qxk_cart_to_ao_axis(const QxShellsDev* sh, const double* in, double* out, lo
 int qxk_oei(const QxShellsDev* sh, double* S, double* T, double* V, hipStream_t s);
 int qxk_oei_deriv(const QxShellsDev* sh, const double* Pd, const double* Wd, double* grad, double* dS, double* dT,
                   double* dV, hipStream_t s);
+// grad += Pd . d/dtv d(T+V)/dR + Wd . d/dtv dS/dR (Cartesian Pd, Wd)
+int qxk_oei_hess_dir(const QxShellsDev* sh, const double* Pd, const double* Wd, const double* tv, double* grad,
+                     hipStream_t s);
 }

@@ -975,14 +975,82 @@ static void deriv_tensor(qx_ctx* c, int coord, double* out, hipStream_t s) {
   if (c->pure) ao_tensor4_from_cart(c, dst, out, nslices, s);
 }
 
+// Tangent (second-derivative) kernels, every part of every class.
+static void add_dert_launches(qx_ctx* c, const QxEpiDev& e, std::vector<Launch>& out) {
+  for (const auto& wh : c->work) {
+    if (wh.w.nquart <= 0) continue;
+    const QxClassEntry& ent = qx_class_table[wh.cls];
+    for (int k = 0; k < ent.ntparts; ++k) {
+      QxArgs a;
+      a.pr = c->pr; a.pp = c->pp; a.w = wh.w; a.e = e; a.boys = c->d_boys;
+      auto fn = ent.dert[k];
+      const long long nq = wh.w.nquart;
+      out.push_back({(double)nq / ent.ntparts, [a, fn](hipStream_t st) {
+                       QxLaunch L{0, st};
+                       return fn(&a, &L);
+                     }});
+    }
+  }
+}
+
+// Index of the coordinate pair (x <= y) among the m(m+1)/2 pairs in
+// generate_multi_index_lookup(m, 2) order.
+static long long pair_index(int x, int y, int m) {
+  return (long long)x * m - (long long)x * (x - 1) / 2 + (y - x);
+}
+
+// Second-derivative tensor: for each nuclear coordinate x the tangent
+// kernels along e_x give d2(ab|cd)/dx dy for every y; slices y >= x go to
+// pair (x, y) (all pairs when want < 0, else only pair `want` into slice 0).
+// Quartets on one atom are skipped, like the first-derivative tensor.
+static void deriv2_tensor(qx_ctx* c, long long want, double* out, hipStream_t s) {
+  const size_t nc = c->ncart;
+  const size_t n4 = nc * nc * nc * nc;
+  const int m = 3 * c->natom;
+  const long long npair = (long long)m * (m + 1) / 2;
+  const long long nslices = want >= 0 ? 1 : npair;
+  std::unique_ptr<DevBuf> tmp(c->pure ? new DevBuf(nslices * n4 * sizeof(double)) : nullptr);
+  double* dst = c->pure ? tmp->d() : out;
+  check_launch(qxk_fill(dst, (long long)(nslices * n4), 0.0, s), "fill");
+  std::vector<double> tv((size_t)m * m, 0.0);
+  std::vector<int> map((size_t)m * m, -1);
+  std::vector<int> xs;
+  for (int x = 0; x < m; ++x) {
+    tv[(size_t)x * m + x] = 1.0;
+    bool any = false;
+    for (int y = x; y < m; ++y) {
+      const long long q = pair_index(x, y, m);
+      if (want < 0) map[(size_t)x * m + y] = (int)q;
+      else if (q == want) map[(size_t)x * m + y] = 0;
+      any = any || map[(size_t)x * m + y] >= 0;
+    }
+    if (any) xs.push_back(x);
+  }
+  DevBuf dtv(tv.size() * sizeof(double)), dmap(map.size() * sizeof(int));
+  QX_HIP(hipMemcpy(dtv.d(), tv.data(), tv.size() * sizeof(double), hipMemcpyHostToDevice));
+  QX_HIP(hipMemcpy(dmap.p, map.data(), map.size() * sizeof(int), hipMemcpyHostToDevice));
+  for (int x : xs) {
+    QxEpiDev e{};
+    e.epi = QX_EPI_DFULL; e.atomic = 1; e.nbf = (int)nc; e.natom = c->natom; e.coord = -1; e.out = dst;
+    e.tv = dtv.d() + (size_t)x * m;
+    e.slot_map = static_cast<const int*>(dmap.p) + (size_t)x * m;
+    std::vector<Launch> v;
+    add_dert_launches(c, e, v);
+    run_launches(c, v, s, "eri second derivative");
+  }
+  if (c->pure) ao_tensor4_from_cart(c, dst, out, nslices, s);
+  QX_HIP(hipStreamSynchronize(s));  // dtv / dmap are freed on return
+}
+
 int qx_eri_deriv_core(qx_ctx* c, int order, double* out, void* stream) {
   if (!c || !out) return QX_EINVAL;
-  if (order != 1) {
-    return fail(c, QX_EUNSUPPORTED, "eri_deriv_core: only deriv_order 1 is built");
+  if (order != 1 && order != 2) {
+    return fail(c, QX_EUNSUPPORTED, "eri_deriv_core: deriv_order 1 and 2 are built");
   }
   return guarded(c, [&]() -> int {
     QX_HIP(hipSetDevice(c->device));
-    deriv_tensor(c, -1, out, (hipStream_t)stream);
+    if (order == 1) deriv_tensor(c, -1, out, (hipStream_t)stream);
+    else deriv2_tensor(c, -1, out, (hipStream_t)stream);
     return QX_OK;
   });
 }
@@ -992,14 +1060,15 @@ int qx_compute_2e_deriv(qx_ctx* c, const int* dv, int len, double* out, void* st
   if (len != 3 * c->natom) {
     return fail(c, QX_EINVAL, "compute_2e_deriv: deriv_vec length must be 3*natom");
   }
-  int order = 0, coord = -1;
+  int order = 0, coord = -1, c2[2] = {-1, -1}, k = 0;
   for (int i = 0; i < len; ++i) {
     if (dv[i] < 0) return fail(c, QX_EINVAL, "compute_2e_deriv: negative deriv_vec entry");
     order += dv[i];
     if (dv[i]) coord = i;
+    for (int r = 0; r < dv[i] && k < 2; ++r) c2[k++] = i;
   }
-  if (order > 1) {
-    return fail(c, QX_EUNSUPPORTED, "compute_2e_deriv: only first-order deriv_vec is built");
+  if (order > 2) {
+    return fail(c, QX_EUNSUPPORTED, "compute_2e_deriv: deriv_vec orders 0, 1 and 2 are built");
   }
   return guarded(c, [&]() -> int {
     QX_HIP(hipSetDevice(c->device));
@@ -1007,9 +1076,20 @@ int qx_compute_2e_deriv(qx_ctx* c, const int* dv, int len, double* out, void* st
     if (order == 0) {
       // the reference's order-0 call returns buffer 0 of a deriv-0 engine, i.e.
       // the integrals, with single-atom quartets skipped (libint_interface.cc:1180-1185)
-      return fail(c, QX_EUNSUPPORTED, "compute_2e_deriv: an all-zero deriv_vec is not built");
+      const size_t nc = c->ncart;
+      const size_t n4 = nc * nc * nc * nc;
+      std::unique_ptr<DevBuf> tmp(c->pure ? new DevBuf(n4 * sizeof(double)) : nullptr);
+      double* dst = c->pure ? tmp->d() : out;
+      check_launch(qxk_fill(dst, (long long)n4, 0.0, s), "fill");
+      QxEpiDev e{};
+      e.epi = QX_EPI_FULL; e.atomic = c->k1 ? 0 : 1; e.nbf = (int)nc; e.natom = c->natom; e.coord = -1;
+      e.out = dst; e.skip_single = 1;
+      for (auto& wh : c->work) check_launch(launch_eri(c, wh.w, wh.cls, e, 0, s, c->pr, c->pp), "compute_2e_deriv");
+      if (c->pure) ao_tensor4_from_cart(c, dst, out, 1, s);
+      return QX_OK;
     }
-    deriv_tensor(c, coord, out, s);
+    if (order == 1) deriv_tensor(c, coord, out, s);
+    else deriv2_tensor(c, pair_index(c2[0], c2[1], len), out, s);
     return QX_OK;
   });
 }
@@ -1142,6 +1222,70 @@ int qx_eri_grad(qx_ctx* c, const double* D, double* grad, int precision, double 
   });
 }
 
+// Forward-over-reverse piece of the Hessian-vector product: the adjoint
+// contraction of the second-derivative integrals along v,
+// out[y] += sum_pqrs Gamma(D)_pqrs d/dv d(pq|rs)/dR_y, Gamma = 2 D_pq D_rs -
+// D_pr D_qs (tangent kernels, integrals recomputed, nothing stored).
+int qx_eri_hess_dir(qx_ctx* c, const double* D, const double* v, double* out, void* stream) {
+  if (!c || !D || !v || !out) return QX_EINVAL;
+  return guarded(c, [&]() -> int {
+    QX_HIP(hipSetDevice(c->device));
+    hipStream_t s = (hipStream_t)stream;
+    const size_t nc = c->ncart, n2 = nc * nc;
+    double* w = begin_ws(c, s);
+    const double* Dcart = D;
+    if (c->pure) {
+      check_launch(qxk_ao_to_cart_mat(&c->shells, D, w + 2 * n2, s), "ao_to_cart");
+      Dcart = w + 2 * n2;
+    }
+    QxEpiDev e{};
+    e.epi = QX_EPI_GRAD; e.nbf = (int)nc; e.natom = c->natom; e.coord = -1; e.D = Dcart; e.grad = out; e.tv = v;
+    std::vector<Launch> L;
+    add_dert_launches(c, e, L);
+    run_launches(c, L, s, "eri_hess_dir");
+    end_ws(c, s);
+    return QX_OK;
+  });
+}
+
+// J and K of the directional derivative of the integrals along v at fixed D:
+// J_pq = sum_rs (d/dv (pq|rs)) D_rs, K_pq = sum_rs (d/dv (pr|qs)) D_rs (the
+// derivative Fock matrix of the coupled-perturbed equations).
+int qx_jk_deriv_dir(qx_ctx* c, const double* D, const double* v, double* J, double* K, void* stream) {
+  if (!c || !D || !v || !J || !K) return QX_EINVAL;
+  return guarded(c, [&]() -> int {
+    QX_HIP(hipSetDevice(c->device));
+    hipStream_t s = (hipStream_t)stream;
+    const size_t nc = c->ncart, n2 = nc * nc;
+    double* w = begin_ws(c, s);
+    double *Jp = w, *Kp = w + n2, *Dc = w + 2 * n2, *Js = w + 3 * n2, *Ks = w + 4 * n2;
+    const double* Dcart = D;
+    if (c->pure) {
+      check_launch(qxk_ao_to_cart_mat(&c->shells, D, Dc, s), "ao_to_cart");
+      Dcart = Dc;
+    }
+    check_launch(qxk_fill(Jp, (long long)n2, 0.0, s), "fill");
+    check_launch(qxk_fill(Kp, (long long)n2, 0.0, s), "fill");
+    QxEpiDev e{};
+    e.epi = QX_EPI_JKD; e.nbf = (int)nc; e.natom = c->natom; e.coord = -1; e.D = Dcart; e.J = Jp; e.K = Kp;
+    e.tv = v;
+    std::vector<Launch> L;
+    add_deriv_launches(c, c->work, e, 0, L);
+    run_launches(c, L, s, "jk_deriv_dir");
+    if (c->pure) {
+      check_launch(qxk_sym_half(Jp, Js, (int)nc, s), "sym");
+      check_launch(qxk_sym_half(Kp, Ks, (int)nc, s), "sym");
+      check_launch(qxk_cart_to_ao_mat(&c->shells, Js, J, s), "cart_to_ao");
+      check_launch(qxk_cart_to_ao_mat(&c->shells, Ks, K, s), "cart_to_ao");
+    } else {
+      check_launch(qxk_sym_half(Jp, J, (int)nc, s), "sym");
+      check_launch(qxk_sym_half(Kp, K, (int)nc, s), "sym");
+    }
+    end_ws(c, s);
+    return QX_OK;
+  });
+}
+
 int qx_oei(qx_ctx* c, double* S, double* T, double* V, void* stream) {
   if (!c || !S || !T || !V) return QX_EINVAL;
   return guarded(c, [&]() -> int {
@@ -1181,19 +1325,51 @@ int qx_oei_grad(qx_ctx* c, const double* Pd, const double* Wd, double* grad, voi
   });
 }
 
-int qx_oei_deriv(qx_ctx* c, double* dS, double* dT, double* dV, void* stream) {
-  if (!c || !dS || !dT || !dV) return QX_EINVAL;
-  if (c->pure) {
-    return fail(c, QX_EUNSUPPORTED, "oei_deriv: Cartesian bases only");
-  }
+// One-electron piece of the Hessian-vector product: grad[3N] +=
+// sum_pq Pd_pq d/dv d(T+V)_pq/dR + Wd_pq d/dv dS_pq/dR (tangent mode of the
+// one-electron gradient kernel; v: 3N device vector).
+int qx_oei_hess_dir(qx_ctx* c, const double* Pd, const double* Wd, const double* v, double* grad, void* stream) {
+  if (!c || !Pd || !Wd || !v || !grad) return QX_EINVAL;
   return guarded(c, [&]() -> int {
     QX_HIP(hipSetDevice(c->device));
     hipStream_t s = (hipStream_t)stream;
-    const long long n = 3LL * c->natom * c->ncart * c->ncart;
-    check_launch(qxk_fill(dS, n, 0.0, s), "fill");
-    check_launch(qxk_fill(dT, n, 0.0, s), "fill");
-    check_launch(qxk_fill(dV, n, 0.0, s), "fill");
-    check_launch(qxk_oei_deriv(&c->shells, nullptr, nullptr, nullptr, dS, dT, dV, s), "oei_deriv");
+    const size_t nc = c->ncart;
+    if (!c->pure) {
+      check_launch(qxk_oei_hess_dir(&c->shells, Pd, Wd, v, grad, s), "oei_hess_dir");
+    } else {
+      DevBuf a(nc * nc * 8), b(nc * nc * 8);
+      check_launch(qxk_ao_to_cart_mat(&c->shells, Pd, a.d(), s), "ao_to_cart");
+      check_launch(qxk_ao_to_cart_mat(&c->shells, Wd, b.d(), s), "ao_to_cart");
+      check_launch(qxk_oei_hess_dir(&c->shells, a.d(), b.d(), v, grad, s), "oei_hess_dir");
+      QX_HIP(hipStreamSynchronize(s));
+    }
+    return QX_OK;
+  });
+}
+
+int qx_oei_deriv(qx_ctx* c, double* dS, double* dT, double* dV, void* stream) {
+  if (!c || !dS || !dT || !dV) return QX_EINVAL;
+  return guarded(c, [&]() -> int {
+    QX_HIP(hipSetDevice(c->device));
+    hipStream_t s = (hipStream_t)stream;
+    const long long m = 3LL * c->natom, nc2 = (long long)c->ncart * c->ncart, nb2 = (long long)c->nbf * c->nbf;
+    std::unique_ptr<DevBuf> tmp(c->pure ? new DevBuf((size_t)(3 * m * nc2) * sizeof(double)) : nullptr);
+    double* cS = c->pure ? tmp->d() : dS;
+    double* cT = c->pure ? tmp->d() + m * nc2 : dT;
+    double* cV = c->pure ? tmp->d() + 2 * m * nc2 : dV;
+    check_launch(qxk_fill(cS, m * nc2, 0.0, s), "fill");
+    check_launch(qxk_fill(cT, m * nc2, 0.0, s), "fill");
+    check_launch(qxk_fill(cV, m * nc2, 0.0, s), "fill");
+    check_launch(qxk_oei_deriv(&c->shells, nullptr, nullptr, nullptr, cS, cT, cV, s), "oei_deriv");
+    if (c->pure) {
+      // every coordinate slice Cartesian -> AO
+      for (long long x = 0; x < m; ++x) {
+        check_launch(qxk_cart_to_ao_mat(&c->shells, cS + x * nc2, dS + x * nb2, s), "cart_to_ao");
+        check_launch(qxk_cart_to_ao_mat(&c->shells, cT + x * nc2, dT + x * nb2, s), "cart_to_ao");
+        check_launch(qxk_cart_to_ao_mat(&c->shells, cV + x * nc2, dV + x * nb2, s), "cart_to_ao");
+      }
+      QX_HIP(hipStreamSynchronize(s));
+    }
     return QX_OK;
   });
 }

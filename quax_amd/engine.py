@@ -219,6 +219,32 @@ class EriEngine:
               self.ctx, "eri_grad")
         return g
 
+    # ------------------------------------------------------------ forward over reverse
+    def eri_hess_dir(self, D, v):
+        """sum_pqrs Gamma(D)_pqrs d/dv d(pq|rs)/dR (3N): the adjoint gradient
+        kernels in forward mode along the geometry direction v."""
+        D, v = self._dev(D), self._dev(v)
+        g = self._zeros(3 * self.natom)
+        check(self.L.qx_eri_hess_dir(self.ctx, ptr(D), ptr(v), ptr(g), self._stream()), self.ctx, "eri_hess_dir")
+        return g
+
+    def jk_deriv_dir(self, D, v):
+        """J and K of the integrals' directional derivative along v at fixed D."""
+        D, v = self._dev(D), self._dev(v)
+        n = self.nbf
+        J, K = self._empty(n, n), self._empty(n, n)
+        check(self.L.qx_jk_deriv_dir(self.ctx, ptr(D), ptr(v), ptr(J), ptr(K), self._stream()), self.ctx,
+              "jk_deriv_dir")
+        return J, K
+
+    def oei_hess_dir(self, Pd, Wd, v):
+        """sum_pq Pd d/dv d(T+V)/dR + Wd d/dv dS/dR (3N)."""
+        Pd, Wd, v = self._dev(Pd), self._dev(Wd), self._dev(v)
+        g = self._zeros(3 * self.natom)
+        check(self.L.qx_oei_hess_dir(self.ctx, ptr(Pd), ptr(Wd), ptr(v), ptr(g), self._stream()), self.ctx,
+              "oei_hess_dir")
+        return g
+
     def oei(self):
         n = self.nbf
         S, T, V = self._empty(n, n), self._empty(n, n), self._empty(n, n)

@@ -11,7 +11,8 @@ Differences, all deliberate:
   throws std::invalid_argument -> ValueError, libint_interface.cc:520).
 * A wrong ``deriv_vec`` length raises ``ValueError`` instead of aborting the
   process (libint_interface.cc:1098).
-* Only first derivatives are built (``LIBINT2_MAX_DERIV_ORDER`` is 1 here).
+* Derivatives up to second order are built (``LIBINT2_MAX_DERIV_ORDER`` is 2
+  here; the reference's value is whatever its Libint build chose).
 * The four basis names must agree (the HF path, ints.py:19-20).
 """
 from __future__ import annotations
@@ -22,7 +23,7 @@ from ..basis import BasisSet
 from ..engine import EriEngine
 from ..molecule import Molecule, read_xyz
 
-LIBINT2_MAX_DERIV_ORDER = 1
+LIBINT2_MAX_DERIV_ORDER = 2
 _F12_TYPES = ("f12", "f12g12", "f12_squared", "f12_double_commutator")
 
 _engine: EriEngine | None = None
@@ -76,7 +77,7 @@ def compute_2e_int(type_: str, beta: float) -> np.ndarray:
 def eri_deriv_core(deriv_order: int) -> np.ndarray:
     """libint_interface.cc:2582-2779: flat (how_many_derivs * nbf^4)."""
     if deriv_order > LIBINT2_MAX_DERIV_ORDER or deriv_order < 1:
-        raise NotImplementedError(f"eri_deriv_core({deriv_order}): only first derivatives are built")
+        raise NotImplementedError(f"eri_deriv_core({deriv_order}): orders 1 to {LIBINT2_MAX_DERIV_ORDER} are built")
     return _get().eri_deriv_core(deriv_order).reshape(-1).cpu().numpy()
 
 
@@ -86,8 +87,8 @@ def compute_2e_deriv(type_: str, beta: float, deriv_vec) -> np.ndarray:
     dv = np.asarray(deriv_vec, dtype=np.int64).reshape(-1)
     if dv.shape[0] != 3 * _natom:
         raise ValueError("Derivative vector incorrect size for this molecule.")
-    if int(dv.sum()) > LIBINT2_MAX_DERIV_ORDER or int(dv.sum()) < 1:
-        raise NotImplementedError("compute_2e_deriv: only first-order deriv_vec is built")
+    if int(dv.sum()) > LIBINT2_MAX_DERIV_ORDER:
+        raise NotImplementedError(f"compute_2e_deriv: deriv_vec orders up to {LIBINT2_MAX_DERIV_ORDER} are built")
     return _get().compute_2e_deriv(dv).reshape(-1).cpu().numpy()
 
 

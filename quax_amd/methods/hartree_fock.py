@@ -220,3 +220,107 @@ def rhf_gradient(engine, aux, precision="fp64", fp32_threshold=1e-8):
     Z = [mol.charge(i) for i in range(mol.natom())]
     gn = nuclear_repulsion_grad(mol.geometry(), Z)
     return (g1 + g2).cpu().numpy() + gn
+
+
+def nuclear_repulsion_hessian(geom, Z):
+    """(3N, 3N) second derivatives of sum_{i<j} Z_i Z_j / |R_i - R_j|."""
+    g = np.asarray(geom, dtype=np.float64).reshape(-1, 3)
+    n = g.shape[0]
+    Hn = np.zeros((3 * n, 3 * n))
+    for i in range(n):
+        for j in range(n):
+            if i == j:
+                continue
+            r = g[i] - g[j]
+            d = np.linalg.norm(r)
+            blk = Z[i] * Z[j] * (3.0 * np.outer(r, r) / d ** 5 - np.eye(3) / d ** 3)
+            Hn[3 * i:3 * i + 3, 3 * i:3 * i + 3] += blk
+            Hn[3 * i:3 * i + 3, 3 * j:3 * j + 3] -= blk
+    return Hn
+
+
+def _cphf_solve(jk2, Cv, Co, denom, rhs, conv, maxit):
+    """Preconditioned conjugate gradients for the RHF orbital response
+    (eps_a - eps_i) U_ai + [Cv^T G(Cv U Co^T + Co U^T Cv^T) Co]_ai = rhs_ai,
+    G(X) = 2 J(X) - K(X): the (symmetric, positive definite at a stable
+    minimum) RHF orbital Hessian, one J/K build per iteration."""
+    import torch
+
+    def op(U):
+        X = Cv @ U @ Co.T
+        return denom * U + Cv.T @ jk2(X + X.T) @ Co
+
+    U = rhs / denom
+    r = rhs - op(U)
+    z = r / denom
+    p = z.clone()
+    rz = torch.sum(r * z)
+    bnorm = float(torch.linalg.norm(rhs)) or 1.0
+    for it in range(maxit):
+        if float(torch.linalg.norm(r)) <= conv * bnorm:
+            return U, it
+        Ap = op(p)
+        alpha = rz / torch.sum(p * Ap)
+        U = U + alpha * p
+        r = r - alpha * Ap
+        z = r / denom
+        rz_new = torch.sum(r * z)
+        p = z + (rz_new / rz) * p
+        rz = rz_new
+    raise RuntimeError(f"CPHF did not converge in {maxit} iterations "
+                       f"(residual {float(torch.linalg.norm(r)) / bnorm:.2e})")
+
+
+def rhf_hessian_vector_product(engine, aux, v, cphf_conv=1e-10, cphf_maxit=200):
+    """H v for the converged RHF energy of ``aux`` (restricted_hartree_fock's
+    aux data), by forward-over-reverse: the total derivative along the
+    geometry direction v of the analytic gradient
+        g = 2 D.dH + sum (2 D_pq D_rs - D_pr D_qs) d(pq|rs) - 2 W.dS + dE_nuc,
+    with W = D F D.  The density response comes from the coupled-perturbed
+    equations (skeleton derivative Fock matrix from the derivative integrals'
+    J/K along v); the terms that differentiate integral derivatives run the
+    adjoint kernels in forward mode (qx_eri_hess_dir, qx_oei_hess_dir), and
+    the response of the two-electron term uses the exact polarisation
+    identity of the quadratic Gamma(D):
+        sum dGamma d(pq|rs) = [grad2(D + dD) - grad2(D - dD)] / 2.
+    No second-derivative tensor is formed.  Returns a (3N,) numpy array."""
+    import torch
+    dev = engine.device
+    dt = torch.float64
+    natom = engine.natom
+    v = np.asarray(v, dtype=np.float64).reshape(-1)
+    if v.shape != (3 * natom,):
+        raise ValueError("v must have 3 * natom entries")
+    vt = torch.as_tensor(v, dtype=dt, device=dev)
+    D, C, eps, nd, S, H = aux["D"], aux["C"], aux["eps"], aux["ndocc"], aux["S"], aux["H"]
+
+    def jk2(X):
+        J, K = engine.jk(X)
+        return 2.0 * J - K
+
+    F = H + jk2(D)
+    W = D @ F @ D
+    dS, dT, dV = engine.oei_deriv()
+    Sv = torch.tensordot(vt, dS, dims=1)
+    Hv = torch.tensordot(vt, dT + dV, dims=1)
+    Jv, Kv = engine.jk_deriv_dir(D, vt)
+    Gv = 2.0 * Jv - Kv
+    Co, Cv = C[:, :nd], C[:, nd:]
+    eo, ev = eps[:nd], eps[nd:]
+    St = C.T @ Sv @ C
+    D0 = -Co @ St[:nd, :nd] @ Co.T
+    F0 = C.T @ (Hv + Gv + jk2(D0)) @ C
+    rhs = -(F0[nd:, :nd] - St[nd:, :nd] * eo[None, :])
+    denom = ev[:, None] - eo[None, :]
+    U, _ = _cphf_solve(jk2, Cv, Co, denom, rhs, cphf_conv, cphf_maxit)
+    X = Cv @ U @ Co.T
+    Dd = D0 + X + X.T
+    Fd = Hv + Gv + jk2(Dd)
+    Wd = Dd @ F @ D + D @ Fd @ D + D @ F @ Dd
+    h = engine.oei_grad(2.0 * Dd, -2.0 * Wd)
+    h = h + engine.oei_hess_dir(2.0 * D, -2.0 * W, vt)
+    h = h + 0.5 * (engine.eri_grad(D + Dd) - engine.eri_grad(D - Dd))
+    h = h + engine.eri_hess_dir(D, vt)
+    mol = engine.molecule
+    Z = [mol.charge(i) for i in range(mol.natom())]
+    return h.cpu().numpy() + nuclear_repulsion_hessian(mol.geometry(), Z) @ v

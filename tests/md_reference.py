@@ -9,6 +9,7 @@ checks, so agreement pins both the arithmetic and the conventions
 """
 from __future__ import annotations
 
+import itertools
 import math
 from functools import lru_cache
 
@@ -245,3 +246,53 @@ def eri_contracted(shells):
                     blk = w * eri_primitive(la, a, A, lb, b, B, lc, c, C, ld, d, D)
                     blocks = blk if blocks is None else blocks + blk
     return blocks
+
+
+def eri_deriv2_contracted(shells):
+    """Second derivatives with respect to pairs of the 12 centre coordinates
+    (centre-major, then x, y, z; pairs i <= j in lexicographic order): array
+    (78, na, nb, nc, nd).  The Gaussian shift identity applied twice -- the
+    second operator acts on the shifted functions the first produced -- to
+    McMurchie-Davidson blocks of the shifted classes."""
+    norms = [normalize(l, e, c) for (l, e, c, _) in shells]
+    ls = [s[0] for s in shells]
+    comps = [cart_components(l) for l in ls]
+    out = np.zeros((78,) + tuple(len(c) for c in comps))
+    exps = [s[1] for s in shells]
+    cen = [np.asarray(s[3], float) for s in shells]
+    pairs = [(i, j) for i in range(12) for j in range(i, 12)]
+
+    def op(terms, k, x, ee):
+        res = []
+        for coef, cv in terms:
+            up = [list(c) for c in cv]
+            up[k][x] += 1
+            res.append((coef * 2 * ee[k], tuple(tuple(c) for c in up)))
+            if cv[k][x] > 0:
+                dn = [list(c) for c in cv]
+                dn[k][x] -= 1
+                res.append((-coef * cv[k][x], tuple(tuple(c) for c in dn)))
+        return res
+
+    for i0, e0 in enumerate(exps[0]):
+        for i1, e1 in enumerate(exps[1]):
+            for i2, e2 in enumerate(exps[2]):
+                for i3, e3 in enumerate(exps[3]):
+                    w = norms[0][i0] * norms[1][i1] * norms[2][i2] * norms[3][i3]
+                    ee = (e0, e1, e2, e3)
+                    cache = {}
+
+                    def val(cv):
+                        lv = tuple(sum(c) for c in cv)
+                        if lv not in cache:
+                            cache[lv] = eri_block_md(lv[0], e0, cen[0], lv[1], e1, cen[1], lv[2], e2, cen[2],
+                                                     lv[3], e3, cen[3])
+                        idx = tuple(_shift_index(lv[t])[cv[t]] for t in range(4))
+                        return cache[lv][idx]
+
+                    for f in itertools.product(*[range(len(c)) for c in comps]):
+                        cv = tuple(comps[t][f[t]] for t in range(4))
+                        for q, (I, J) in enumerate(pairs):
+                            terms = op(op([(1.0, cv)], I // 3, I % 3, ee), J // 3, J % 3, ee)
+                            out[(q,) + f] += w * sum(c * val(v) for c, v in terms)
+    return out

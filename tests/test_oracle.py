@@ -189,6 +189,85 @@ def _cluster3():
     return mol, BasisSet("t", shells)
 
 
+_CLASSES2 = [(0, 0, 0, 0), (1, 0, 0, 0), (1, 0, 1, 0), (1, 1, 1, 0), (2, 0, 1, 0), (2, 1, 0, 0), (1, 1, 1, 1),
+             (2, 0, 2, 0)]
+
+
+@pytest.mark.parametrize("cls", _CLASSES2, ids=lambda c: "".join(map(str, c)))
+def test_second_derivatives_match_mcmurchie_davidson(cls):
+    """All 78 second derivatives w.r.t. pairs of the 12 centre coordinates
+    for classes up to (pp|pp) and (ds|ds): the oracle's HGP with the shift
+    identity applied twice against the same identity on independent
+    McMurchie-Davidson blocks."""
+    rng = np.random.default_rng(300 + sum(10 ** i * l for i, l in enumerate(cls)))
+    mol = Molecule(["C", "H", "O", "N"], rng.uniform(-1.2, 1.2, (4, 3)))
+    shells = [Shell(l, a, [float(rng.uniform(0.3, 2.5))], [1.0]) for a, l in enumerate(cls)]
+    ob = oracle.OracleBasis(mol, BasisSet("t", shells))
+    d2 = ob.shell_quartet_deriv2(0, 1, 2, 3)
+    ref = md.eri_deriv2_contracted([(s.l, s.exps, s.coefs, mol.geometry()[s.atom]) for s in shells])
+    assert np.abs(d2 - ref).max() < 1e-12 * max(1.0, np.abs(ref).max())
+
+
+def test_second_derivative_invariance_and_fd():
+    """Each row of centre second derivatives sums to zero over the four
+    centres (translational invariance), and the 78 blocks are central
+    differences of the first derivatives (Richardson-extrapolated)."""
+    mol, bs = _cluster3()
+    q = (2, 4, 7, 5)   # d p | s p on atoms 0, 1, 2, 1
+    ob = oracle.OracleBasis(mol, bs)
+    d2 = ob.shell_quartet_deriv2(*q)
+    full = np.zeros((12, 12) + d2.shape[1:])
+    k = 0
+    for i in range(12):
+        for j in range(i, 12):
+            full[i, j] = full[j, i] = d2[k]
+            k += 1
+    inv = full[:, 0:3] + full[:, 3:6] + full[:, 6:9] + full[:, 9:12]
+    assert np.abs(inv).max() < 1e-12 * max(1, np.abs(d2).max())
+    # d/dR(atom 2, z) of the first derivatives: atom 2 carries centre C only
+    geom = mol.geometry()
+
+    def d1(h):
+        gp, gm = geom.copy(), geom.copy()
+        gp[2, 2] += h
+        gm[2, 2] -= h
+        fp = oracle.OracleBasis(Molecule(["C", "H", "O"], gp), bs).shell_quartet_deriv1(*q)
+        fm = oracle.OracleBasis(Molecule(["C", "H", "O"], gm), bs).shell_quartet_deriv1(*q)
+        return (fp - fm) / (2 * h)
+
+    fd = (4 * d1(5e-5) - d1(1e-4)) / 3
+    an = full[:, 3 * 2 + 2]
+    assert np.abs(fd - an).max() < 1e-8 * max(1, np.abs(an).max())
+
+
+def test_eri_deriv_core2_semantics():
+    """eri_deriv_core(2) (water/STO-3G, 45 slices): slices equal
+    compute_2e_deriv for second-order vectors, single-atom quartets are
+    zero, and the slices are central differences of eri_deriv_core(1)."""
+    mol = workloads.water_sto3g()
+    bs = BasisSet.build(mol, "sto-3g")
+    ob = oracle.OracleBasis(mol, bs)
+    d2 = ob.eri_deriv_core2(1e-14)
+    assert d2.shape == (45, 7, 7, 7, 7)
+    pairs = [(x, y) for x in range(9) for y in range(x, 9)]
+    for (x, y) in ((0, 0), (1, 5), (4, 8)):
+        dv = np.zeros(9, dtype=int)
+        dv[x] += 1
+        dv[y] += 1
+        assert np.abs(ob.compute_2e_deriv2(1e-14, dv) - d2[pairs.index((x, y))]).max() < 1e-15
+    assert np.all(d2[:, 0, 0, 0, 0] == 0.0)   # O 1s quartet: one atom
+    xyz0 = np.asarray(mol.geometry(), dtype=np.float64).reshape(-1)
+    for x in (0, 4, 8):
+        g = []
+        for s in (+1, -1):
+            xx = xyz0.copy()
+            xx[x] += s * 1e-4
+            g.append(oracle.OracleBasis(Molecule(["O", "H", "H"], xx.reshape(-1, 3)), bs).eri_deriv_core1(1e-14))
+        fd = (g[0] - g[1]) / 2e-4
+        for y in range(9):
+            assert np.abs(fd[y] - d2[pairs.index((min(x, y), max(x, y)))]).max() < 2e-8
+
+
 def test_derivative_translational_invariance_and_fd():
     mol, bs = _cluster3()
     ob = oracle.OracleBasis(mol, bs)
