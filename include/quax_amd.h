@@ -206,9 +206,23 @@ int qx_partial_tei_transformation(const double* G, const int* n, const double* C
                                   void* stream);
 const char* qx_gemm_last_error(void);
 
-/* Mixed-precision split of the canonical screened quartets at
- * fp32_threshold: contracted quartets evaluated in FP64 and in FP32. */
+/* Mixed precision (no reference counterpart).  With QX_PREC_MIXED a pass
+ * evaluates, in the classes its FP32 policy admits and in which at least a
+ * quarter of the primitive quartets have a Schwarz bound below
+ * fp32_threshold, those quartets with FP32 recursion and the rest in FP64
+ * (two launches over one quartet list, each masked to its part); every other
+ * class runs in FP64.
+ * qx_mixed_quartets: contracted quartets above / below the threshold over
+ * every class.  qx_mixed_pass_quartets: FP64 / FP32 contracted quartets of
+ * a pass (0 = J/K, 1 = adjoint gradient; -1 = as qx_mixed_quartets).
+ * qx_set_fp32_classes: per-class policy of a pass, flags[qx_num_classes()]
+ * (nonzero = FP32 allowed), or the built-in default (from timings of FP32
+ * against FP64 kernels per class) when flags is NULL; qx_get_fp32_classes
+ * reads it. */
 int qx_mixed_quartets(qx_ctx* ctx, double fp32_threshold, long long* n_fp64, long long* n_fp32);
+int qx_mixed_pass_quartets(qx_ctx* ctx, double fp32_threshold, int pass, long long* n_fp64, long long* n_fp32);
+int qx_set_fp32_classes(qx_ctx* ctx, int pass, const int* flags);
+int qx_get_fp32_classes(const qx_ctx* ctx, int pass, int* flags);
 
 /* Independent class kernels of a J/K or gradient pass are spread over
  * nstreams streams (1..8; default 4): the caller's stream plus side streams

@@ -72,6 +72,9 @@ SIGNATURES = {
     "qx_profile_slots": (_i, []),
     "qx_profile_slot": (_i, [_i, _ip, _ip, _ip, _ip]),
     "qx_mixed_quartets": (_i, [_vp, ctypes.c_double, ctypes.POINTER(_ll), ctypes.POINTER(_ll)]),
+    "qx_mixed_pass_quartets": (_i, [_vp, ctypes.c_double, _i, ctypes.POINTER(_ll), ctypes.POINTER(_ll)]),
+    "qx_set_fp32_classes": (_i, [_vp, _i, _vp]),
+    "qx_get_fp32_classes": (_i, [_vp, _i, _vp]),
     "qx_set_streams": (_i, [_vp, _i]),
     "qx_class_info": (_i, [_i, _ip, _ip]),
     "qx_class_quartets": (_i, [_vp, ctypes.POINTER(_ll), ctypes.POINTER(_ll)]),

@@ -59,6 +59,14 @@ if os.path.exists(_BUDGET_FILE):
         DERIV_CSPLIT = {tuple(int(ch) for ch in k): int(v) for k, v in json.load(_fh).get("csplit", {}).items()}
 _CSPLIT_OVERRIDE = os.environ.get("QX_CSPLIT")
 
+# classes whose mixed-precision passes may use FP32 (J/K, gradient): those
+# whose FP32 kernels timed faster than their FP64 ones (fewer spills)
+FP32_CLASSES = {"jk": set(), "grad": set()}
+if os.path.exists(_BUDGET_FILE):
+    with open(_BUDGET_FILE) as _fh:
+        _f = json.load(_fh).get("fp32", {})
+        FP32_CLASSES = {k: {tuple(int(ch) for ch in c) for c in _f.get(k, [])} for k in ("jk", "grad")}
+
 
 def deriv_csplit(l):
     """c-component ranges per (a, b) part for classes whose (a, b) parts
@@ -739,6 +747,11 @@ def main():
         D.append(f"  {{{cls[0]}, {cls[1]}, {cls[2]}, {cls[3]}, qx_launch_e{nm}, {len(jkf[cls])}, {{{jl}}}, {{{jfs}}}, "
                  f"{nparts[cls]}, {{{dl}}}, {ntparts[cls]}, {{{tl}}}, {len(groups[cls])}, {{{gl}}}, {ef:.1f}, {{{pfs}}}, {{{pcs}}}, {{{gfs}}}, "
                  f"{{{gps}}}}},")
+    D.append("};")
+    D.append("// per class: mixed-precision J/K and gradient passes may use FP32")
+    D.append("static const unsigned char qx_fp32_default[QX_NUM_CLASSES][2] = {")
+    for cls in classes:
+        D.append(f"  {{{int(cls in FP32_CLASSES['jk'])}, {int(cls in FP32_CLASSES['grad'])}}},")
     D.append("};")
     with open(os.path.join(outdir, "qx_classes.inc"), "w") as fh:
         fh.write("\n".join(D) + "\n")

@@ -68,6 +68,12 @@ struct QxWorkDev {         // one (bra class, ket class) quartet list
   long long nquart;
   long long store_base;      // element offset of this list in the store buffer
   int diag;                  // kets are the pseudo pairs of the bra's own pair
+  // Mixed precision over one list: the first nk64[row] kets of each bra row
+  // (Schwarz bound >= the FP32 threshold; kets are sorted by descending
+  // bound) run in FP64, the rest in FP32.  sel 0: every quartet; 1: the FP64
+  // kets only; 2: the FP32 kets only.
+  const int* nk64;
+  int sel;
 };
 
 struct QxEpiDev {
@@ -195,7 +201,9 @@ __device__ __forceinline__ QxTanPair qx_tan_pair(const QxArgs& A_, int P, int u)
   return r;
 }
 
-__device__ __forceinline__ void qx_locate(const QxArgs& A_, long long t, int& u1, int& u2, int& P1, int& P2) {
+// Quartet t of the list: bra / ket pseudo pairs u1 / u2 and their pairs.
+// Returns whether the quartet belongs to this launch's precision (w.sel).
+__device__ __forceinline__ bool qx_locate(const QxArgs& A_, long long t, int& u1, int& u2, int& P1, int& P2) {
   const long long* pre = A_.w.prefix;
   int lo = 0, hi = A_.w.nbra;  // pre[lo] <= t < pre[hi]
   while (hi - lo > 1) {
@@ -207,6 +215,13 @@ __device__ __forceinline__ void qx_locate(const QxArgs& A_, long long t, int& u1
   P1 = A_.pp.pair[u1];
   u2 = A_.w.diag ? A_.pr.pp_first[P1] + local : A_.w.ket_off + (A_.w.kstart ? A_.w.kstart[lo] : 0) + local;
   P2 = A_.pp.pair[u2];
+  return A_.w.sel == 0 || ((local < A_.w.nk64[lo]) == (A_.w.sel == 1));
+}
+
+// true when no thread of the block holds a quartet of this launch's
+// precision (block-uniform; every thread must call it)
+__device__ __forceinline__ bool qx_block_idle(const QxArgs& A_, bool active) {
+  return A_.w.sel != 0 && !__syncthreads_or(active ? 1 : 0);
 }
 
 __device__ __forceinline__ void qx_atomic_add(double* p, double v) { unsafeAtomicAdd(p, v); }
@@ -593,9 +608,11 @@ __device__ __forceinline__ void qx_epi_djk(
   __global__ __launch_bounds__(QX_BLOCK) void qx_kj_##NAME(QxArgs A_) {                           \
     HIP_DYNAMIC_SHARED(double, qx_sk)                                                             \
     const long long t = (long long)blockIdx.x * QX_BLOCK + threadIdx.x;                          \
-    const bool active = t < A_.w.nquart;                                                          \
+    bool active = t < A_.w.nquart;                                                                \
     int u1 = 0, u2 = 0, P1 = -1, P2 = -1;                                                         \
-    if (active) qx_locate(A_, t, u1, u2, P1, P2);                                                 \
+    if (active) active = qx_locate(A_, t, u1, u2, P1, P2);                                        \
+    if (qx_block_idle(A_, active)) return;                                                        \
+    if (!active) P1 = P2 = -1;                                                                    \
     const int n_ = A_.e.nbf;                                                                      \
     double* sk = nullptr;                                                                         \
     int ra = 0, rb = 0;                                                                           \
@@ -737,14 +754,16 @@ template <class Part>
 __device__ __forceinline__ void qx_grad_body(const QxArgs& A_, double* sg, Part part) {
   const int ng = 3 * A_.e.natom;
   const bool lds = ng <= QX_GRAD_LDS;
+  const long long t = (long long)blockIdx.x * QX_BLOCK + threadIdx.x;
+  int u1 = 0, u2 = 0, P1 = -1, P2 = -1;
+  bool active = t < A_.w.nquart;
+  if (active) active = qx_locate(A_, t, u1, u2, P1, P2);
+  if (qx_block_idle(A_, active)) return;
   if (lds) {
     for (int i = threadIdx.x; i < ng; i += QX_BLOCK) sg[i] = 0.0;
     __syncthreads();
   }
-  const long long t = (long long)blockIdx.x * QX_BLOCK + threadIdx.x;
-  if (t < A_.w.nquart) {
-    int u1, u2, P1, P2;
-    qx_locate(A_, t, u1, u2, P1, P2);
+  if (active) {
     if (!qx_single_atom(A_, P1, P2)) {
       double g[QX_GRAD_NCOMB];
 #pragma unroll

@@ -35,6 +35,8 @@ struct QxWorkHost {
   long long prefix_off;    // offset of this list's prefix in ctx->d_prefix
 };
 
+#define QX_NUM_CLASSES_MAX 21  // canonical classes with l <= 2
+
 struct qx_ctx {
   int device = 0;
   int natom = 0;
@@ -109,19 +111,24 @@ struct qx_ctx {
   hipEvent_t ws_ev = nullptr;
   bool ws_used = false;
 
-  // mixed precision: quartet lists split at one Schwarz-bound threshold
+  // mixed precision: per class and pass (0 J/K, 1 gradient) whether quartets
+  // below the threshold run in FP32 (classes whose FP32 kernels are faster)
+  unsigned char fp32_ok[2][QX_NUM_CLASSES_MAX] = {};
+  // the quartet lists with each bra row's FP64 ket count at one Schwarz-bound
+  // threshold (kets are sorted by descending bound, so the FP64 kets of a row
+  // are its first nk64 kets)
   struct Split {
     double thr = -1.0;
-    std::vector<QxWorkHost> w64, w32;  // FP64 (bound >= thr) and FP32 lists
-    std::vector<void*> bufs;           // prefix / kstart arrays of both
-    long long nq64 = 0, nq32 = 0;      // contracted quartets in each
+    std::vector<QxWorkHost> wm;          // c->work with w.nk64 set
+    std::vector<long long> pq64, pq32;   // per list: pseudo-pair quartets above / below thr
+    std::vector<long long> cq64, cq32;   // per list: contracted quartets above / below thr
+    int* d_nk64 = nullptr;
   } split;
 };
 
 // profile slots: array-form ERI kernel of class c -> QX_SLOT_ERI(c, fp32);
 // derivative part / group k of class c -> QX_SLOT_DERIV(c, k, fp32); J/K
 // part k of class c -> QX_SLOT_JK(c, k, fp32)
-#define QX_NUM_CLASSES_MAX 21
 #define QX_SLOT_ERI(c, f32) ((f32) * QX_NUM_CLASSES_MAX + (c))
 #define QX_SLOT_DERIV(c, k, f32) (2 * QX_NUM_CLASSES_MAX + ((f32) * QX_NUM_CLASSES_MAX + (c)) * QX_DPARTS_CAP + (k))
 #define QX_SLOT_JK0 (2 * QX_NUM_CLASSES_MAX * (1 + QX_DPARTS_CAP))

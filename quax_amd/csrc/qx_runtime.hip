@@ -527,82 +527,52 @@ void end_ws(qx_ctx* c, hipStream_t s) {
   c->ws_used = true;
 }
 
-// Mixed precision: the canonical quartet lists split at a Schwarz-bound
-// threshold.  Kets of each bra pair are sorted by descending Q, so for bra i
-// the FP64 kets (Q_i Q_j >= thr^2) are a prefix of its admissible kets and
-// the FP32 kets the rest; the FP32 list starts each bra's kets at kstart.
+// Mixed precision: for every quartet list, the number of FP64 kets of each
+// bra row at a Schwarz-bound threshold.  Kets of each bra pair are sorted by
+// descending Q, so for bra i the FP64 kets (Q_i Q_j >= thr^2) are a prefix of
+// its admissible kets; a masked launch (w.sel) then runs either part of the
+// unchanged list, keeping its block -> bra-row layout.
 void build_split(qx_ctx* c, double thr) {
   auto& S = c->split;
   if (S.thr == thr) return;
-  QX_HIP(hipDeviceSynchronize());  // no kernel may still read the old lists
-  for (void* p : S.bufs) (void)hipFree(p);
+  QX_HIP(hipDeviceSynchronize());  // no kernel may still read the old counts
+  if (S.d_nk64) (void)hipFree(S.d_nk64);
   S = qx_ctx::Split{};
   const double tol2 = c->tol * c->tol, thr2 = thr * thr;
-  const int nclass = pair_class(QX_LMAX_SUPPORTED, QX_LMAX_SUPPORTED) + 1;
-  std::vector<long long> pre64, pre32;
-  std::vector<int> kst, k64st;  // per bra pseudo pair: first FP32 / first FP64 ket
-  std::vector<long long> off64, off32, offk;
-  for (int X = 0; X < nclass; ++X)
-    for (int Y = 0; Y <= X; ++Y) {
-      const int bo = c->class_pair_off[X], bn = c->class_pair_cnt[X];
-      const int ko = c->class_pair_off[Y], kn = c->class_pair_cnt[Y];
-      if (bn == 0 || kn == 0) continue;
-      std::vector<long long> kcum(kn + 1, 0);
-      for (int j = 0; j < kn; ++j) kcum[j + 1] = kcum[j] + c->h_pp_n[ko + j];
-      QxWorkHost w{};
-      w.cls = class_index(X, Y);
-      w.bra_class = X; w.ket_class = Y;
-      const int u0 = c->h_pp_first[bo];
-      const int u1 = c->h_pp_first[bo + bn - 1] + c->h_pp_n[bo + bn - 1];
-      w.w.bra_off = u0; w.w.nbra = u1 - u0;
-      w.w.ket_off = c->h_pp_first[ko];
-      w.w.nket = (int)kcum[kn];
-      w.w.diag = 0;
-      QxWorkHost w32 = w;
-      off64.push_back((long long)pre64.size());
-      off32.push_back((long long)pre32.size());
-      offk.push_back((long long)kst.size());
-      pre64.push_back(0);
-      pre32.push_back(0);
-      long long run64 = 0, run32 = 0;
-      for (int i = 0; i < bn; ++i) {
-        int ss, kc, ss2, k64;
-        ket_window(c, X, Y, i, tol2, ss, kc);
-        ket_window(c, X, Y, i, thr2, ss2, k64);
-        k64 = std::min(kc, k64);
-        S.nq64 += k64;
-        S.nq32 += kc - k64;
-        for (int k = 0; k < c->h_pp_n[bo + i]; ++k) {
-          run64 += kcum[ss + k64] - kcum[ss];
-          run32 += kcum[ss + kc] - kcum[ss + k64];
-          pre64.push_back(run64);
-          pre32.push_back(run32);
-          k64st.push_back((int)kcum[ss]);
-          kst.push_back((int)kcum[ss + k64]);
-        }
+  std::vector<int> nk;                 // per list, per bra row
+  std::vector<size_t> off;
+  for (const auto& wh : c->work) {
+    const int X = wh.bra_class, Y = wh.ket_class;
+    const int bo = c->class_pair_off[X], bn = c->class_pair_cnt[X];
+    const int ko = c->class_pair_off[Y], kn = c->class_pair_cnt[Y];
+    std::vector<long long> kcum(kn + 1, 0);
+    for (int j = 0; j < kn; ++j) kcum[j + 1] = kcum[j] + c->h_pp_n[ko + j];
+    off.push_back(nk.size());
+    long long p64 = 0, p32 = 0, q64 = 0, q32 = 0;
+    for (int i = 0; i < bn; ++i) {
+      int ss, kc, ss2, k64;
+      ket_window(c, X, Y, i, tol2, ss, kc);
+      ket_window(c, X, Y, i, thr2, ss2, k64);
+      k64 = std::min(kc, k64);
+      q64 += k64;
+      q32 += kc - k64;
+      const long long r64 = kcum[ss + k64] - kcum[ss], rall = kcum[ss + kc] - kcum[ss];
+      for (int k = 0; k < c->h_pp_n[bo + i]; ++k) {
+        nk.push_back((int)r64);
+        p64 += r64;
+        p32 += rall - r64;
       }
-      w.w.nquart = run64;
-      w32.w.nquart = run32;
-      S.w64.push_back(w);
-      S.w32.push_back(w32);
     }
-  auto up = [&](const void* src, size_t bytes) {
-    void* d = nullptr;
-    QX_HIP(hipMalloc(&d, std::max<size_t>(bytes, 8)));
-    S.bufs.push_back(d);
-    if (bytes) QX_HIP(hipMemcpy(d, src, bytes, hipMemcpyHostToDevice));
-    return d;
-  };
-  const long long* d64 = static_cast<const long long*>(up(pre64.data(), pre64.size() * sizeof(long long)));
-  const long long* d32 = static_cast<const long long*>(up(pre32.data(), pre32.size() * sizeof(long long)));
-  const int* dk = static_cast<const int*>(up(kst.data(), kst.size() * sizeof(int)));
-  const int* dk64 = static_cast<const int*>(up(k64st.data(), k64st.size() * sizeof(int)));
-  for (size_t i = 0; i < S.w64.size(); ++i) {
-    S.w64[i].w.prefix = d64 + off64[i];
-    S.w64[i].w.kstart = c->fragmented ? dk64 + offk[i] : nullptr;
-    S.w32[i].w.prefix = d32 + off32[i];
-    S.w32[i].w.kstart = dk + offk[i];
+    if ((long long)(nk.size() - off.back()) != wh.w.nbra)
+      throw QxError{QX_EHIP, "mixed-precision rows do not match the quartet list"};
+    if (p64 + p32 != wh.w.nquart) throw QxError{QX_EHIP, "mixed-precision split lost quartets"};
+    S.pq64.push_back(p64); S.pq32.push_back(p32);
+    S.cq64.push_back(q64); S.cq32.push_back(q32);
   }
+  QX_HIP(hipMalloc(&S.d_nk64, std::max<size_t>(nk.size(), 1) * sizeof(int)));
+  if (!nk.empty()) QX_HIP(hipMemcpy(S.d_nk64, nk.data(), nk.size() * sizeof(int), hipMemcpyHostToDevice));
+  S.wm = c->work;
+  for (size_t i = 0; i < S.wm.size(); ++i) S.wm[i].w.nk64 = S.d_nk64 + off[i];
   S.thr = thr;
 }
 
@@ -717,6 +687,8 @@ static int init_impl(int natom, const double* xyz_bohr, const double* nuclear_ch
   const int rc = guarded(nullptr, [&]() -> int {
     try {
       c = new qx_ctx();
+      for (int k = 0; k < QX_NUM_CLASSES; ++k)
+        for (int pass = 0; pass < 2; ++pass) c->fp32_ok[pass][k] = qx_fp32_default[k][pass];
       init_body(c, natom, xyz_bohr, nuclear_charges, nshell, shell_l, shell_atom, shell_nprim, exps, coefs, pure,
                 ints_tol, device, atom_fragment);
     } catch (...) {
@@ -820,7 +792,7 @@ int qx_finalize(qx_ctx* c) {
   for (auto t : c->side) (void)hipStreamDestroy(t);
   for (auto e : c->fork_ev) (void)hipEventDestroy(e);
   if (c->ws_ev) (void)hipEventDestroy(c->ws_ev);
-  for (void* p : c->split.bufs) (void)hipFree(p);
+  if (c->split.d_nk64) (void)hipFree(c->split.d_nk64);
   for (void* p : c->allocs) (void)hipFree(p);
   delete c;
   return QX_OK;
@@ -912,46 +884,83 @@ int qx_compute_2e_int(qx_ctx* c, double* out, void* stream) {
   });
 }
 
-// Derivative kernels (every part of every class) over one set of lists.
-static void add_deriv_launches(qx_ctx* c, const std::vector<QxWorkHost>& work, const QxEpiDev& e, int fp32,
-                               std::vector<Launch>& out) {
-  for (const auto& wh : work) {
-    if (wh.w.nquart <= 0) continue;
-    const QxClassEntry& ent = qx_class_table[wh.cls];
-    // adjoint gradient: one launch per group of parts; tensor: one per part
-    const bool grad = e.epi == QX_EPI_GRAD;
-    const int nl = grad ? ent.ngroups : ent.nparts;
-    for (int k = 0; k < nl; ++k) {
-      QxArgs a;
-      a.pr = c->pr; a.pp = c->pp; a.w = wh.w; a.e = e; a.boys = c->d_boys;
-      const int slot = QX_SLOT_DERIV(wh.cls, k, fp32 ? 1 : 0);
-      auto fn = grad ? ent.grad[k] : ent.deri[k];
-      const long long nq = wh.w.nquart;
-      const double fl = grad ? ent.group_flops[k] : ent.part_flops[k];
-      out.push_back({(double)nq * fl, [c, a, fn, fp32, slot, nq](hipStream_t st) {
-                       QxLaunch L{fp32, st};
-                       return timed_launch(c, slot, nq, st, [&] { return fn(&a, &L); });
-                     }});
-    }
+// Derivative kernels (every part) of one quartet list; sel / nq: the masked
+// part of the list the launch evaluates and its pseudo-pair quartet count.
+static void add_deriv_list(qx_ctx* c, const QxWorkHost& wh, const QxEpiDev& e, int fp32, int sel, long long nq,
+                           std::vector<Launch>& out) {
+  if (wh.w.nquart <= 0 || nq <= 0) return;
+  const QxClassEntry& ent = qx_class_table[wh.cls];
+  // adjoint gradient: one launch per group of parts; tensor: one per part
+  const bool grad = e.epi == QX_EPI_GRAD;
+  const int nl = grad ? ent.ngroups : ent.nparts;
+  for (int k = 0; k < nl; ++k) {
+    QxArgs a;
+    a.pr = c->pr; a.pp = c->pp; a.w = wh.w; a.e = e; a.boys = c->d_boys;
+    a.w.sel = sel;
+    const int slot = QX_SLOT_DERIV(wh.cls, k, fp32 ? 1 : 0);
+    auto fn = grad ? ent.grad[k] : ent.deri[k];
+    const double fl = grad ? ent.group_flops[k] : ent.part_flops[k];
+    out.push_back({(double)nq * fl, [c, a, fn, fp32, slot, nq](hipStream_t st) {
+                     QxLaunch L{fp32, st};
+                     return timed_launch(c, slot, nq, st, [&] { return fn(&a, &L); });
+                   }});
   }
 }
 
-// J/K kernels (every part of every class) over one set of lists.
+// J/K kernels (every part) of one quartet list, as add_deriv_list.
+static void add_jk_list(qx_ctx* c, const QxWorkHost& wh, const QxEpiDev& e, int fp32, int sel, long long nq,
+                        std::vector<Launch>& out) {
+  if (wh.w.nquart <= 0 || nq <= 0) return;
+  const QxClassEntry& ent = qx_class_table[wh.cls];
+  for (int k = 0; k < ent.njk; ++k) {
+    QxArgs a;
+    a.pr = c->pr; a.pp = c->pp; a.w = wh.w; a.e = e; a.boys = c->d_boys;
+    a.w.sel = sel;
+    const int slot = QX_SLOT_JK(wh.cls, k, fp32 ? 1 : 0);
+    auto fn = ent.jk[k];
+    out.push_back({(double)nq * ent.jk_flops[k], [c, a, fn, fp32, slot, nq](hipStream_t st) {
+                     QxLaunch L{fp32, st};
+                     return timed_launch(c, slot, nq, st, [&] { return fn(&a, &L); });
+                   }});
+  }
+}
+
+static void add_deriv_launches(qx_ctx* c, const std::vector<QxWorkHost>& work, const QxEpiDev& e, int fp32,
+                               std::vector<Launch>& out) {
+  for (const auto& wh : work) add_deriv_list(c, wh, e, fp32, 0, wh.w.nquart, out);
+}
+
 static void add_jk_launches(qx_ctx* c, const std::vector<QxWorkHost>& work, const QxEpiDev& e, int fp32,
                             std::vector<Launch>& out) {
-  for (const auto& wh : work) {
-    if (wh.w.nquart <= 0) continue;
-    const QxClassEntry& ent = qx_class_table[wh.cls];
-    for (int k = 0; k < ent.njk; ++k) {
-      QxArgs a;
-      a.pr = c->pr; a.pp = c->pp; a.w = wh.w; a.e = e; a.boys = c->d_boys;
-      const int slot = QX_SLOT_JK(wh.cls, k, fp32 ? 1 : 0);
-      auto fn = ent.jk[k];
-      const long long nq = wh.w.nquart;
-      out.push_back({(double)nq * ent.jk_flops[k], [c, a, fn, fp32, slot, nq](hipStream_t st) {
-                       QxLaunch L{fp32, st};
-                       return timed_launch(c, slot, nq, st, [&] { return fn(&a, &L); });
-                     }});
+  for (const auto& wh : work) add_jk_list(c, wh, e, fp32, 0, wh.w.nquart, out);
+}
+
+// Mixed precision, per quartet list i of the split (pass 0 J/K, 1
+// gradient): 0 = FP64 kernels, 1 = FP32 kernels, 2 = two launches masked to
+// the FP64 and FP32 kets.  FP32 is used in the classes whose policy admits it
+// and only when at least QX_MIXED_MIN_FRAC of the list's primitive quartets
+// are below the threshold: a masked launch pair runs every wave that holds
+// quartets of both kinds twice, which a small FP32 share does not repay.
+#define QX_MIXED_MIN_FRAC 0.25
+static int mixed_mode(const qx_ctx* c, size_t i, int pass) {
+  const auto& S = c->split;
+  const long long p64 = S.pq64[i], p32 = S.pq32[i];
+  if (!c->fp32_ok[pass][S.wm[i].cls] || p32 < QX_MIXED_MIN_FRAC * (double)(p64 + p32)) return 0;
+  return p64 == 0 ? 1 : 2;
+}
+
+static void add_mixed_launches(qx_ctx* c, double thr, int pass, const QxEpiDev& e, std::vector<Launch>& out) {
+  build_split(c, thr);
+  const auto& S = c->split;
+  auto add = pass == 0 ? add_jk_list : add_deriv_list;
+  for (size_t i = 0; i < S.wm.size(); ++i) {
+    const QxWorkHost& wh = S.wm[i];
+    switch (mixed_mode(c, i, pass)) {
+      case 0: add(c, wh, e, 0, 0, wh.w.nquart, out); break;
+      case 1: add(c, wh, e, 1, 0, wh.w.nquart, out); break;
+      default:
+        add(c, wh, e, 0, 1, S.pq64[i], out);
+        add(c, wh, e, 1, 2, S.pq32[i], out);
     }
   }
 }
@@ -1116,9 +1125,7 @@ int qx_jk_build(qx_ctx* c, const double* D, double* J, double* K, int precision,
     e.krows = 1;  // each class kernel decides whether its K' rows fit in LDS
     std::vector<Launch> v;
     if (precision == QX_PREC_MIXED) {
-      build_split(c, thr);
-      add_jk_launches(c, c->split.w64, e, 0, v);
-      add_jk_launches(c, c->split.w32, e, 1, v);
+      add_mixed_launches(c, thr, 0, e, v);
     } else {
       add_jk_launches(c, c->work, e, 0, v);
     }
@@ -1210,9 +1217,7 @@ int qx_eri_grad(qx_ctx* c, const double* D, double* grad, int precision, double 
     e.epi = QX_EPI_GRAD; e.nbf = (int)nc; e.natom = c->natom; e.coord = -1; e.D = Dcart; e.grad = grad;
     std::vector<Launch> v;
     if (precision == QX_PREC_MIXED) {
-      build_split(c, thr);
-      add_deriv_launches(c, c->split.w64, e, 0, v);
-      add_deriv_launches(c, c->split.w32, e, 1, v);
+      add_mixed_launches(c, thr, 1, e, v);
     } else {
       add_deriv_launches(c, c->work, e, 0, v);
     }
@@ -1464,14 +1469,37 @@ int qx_profile_slot(int slot, int* kind, int* cls, int* part, int* fp32) {
 }
 
 int qx_mixed_quartets(qx_ctx* c, double thr, long long* n64, long long* n32) {
-  if (!c || !(thr >= 0.0 && std::isfinite(thr))) return QX_EINVAL;
+  return qx_mixed_pass_quartets(c, thr, -1, n64, n32);
+}
+
+int qx_mixed_pass_quartets(qx_ctx* c, double thr, int pass, long long* n64, long long* n32) {
+  if (!c || !(thr >= 0.0 && std::isfinite(thr)) || pass < -1 || pass > 1) return QX_EINVAL;
   return guarded(c, [&]() -> int {
     QX_HIP(hipSetDevice(c->device));
     build_split(c, thr);
-    if (n64) *n64 = c->split.nq64;
-    if (n32) *n32 = c->split.nq32;
+    const auto& S = c->split;
+    long long a = 0, b = 0;
+    for (size_t i = 0; i < S.wm.size(); ++i) {
+      const bool f32 = pass < 0 || mixed_mode(c, i, pass) != 0;
+      a += f32 ? S.cq64[i] : S.cq64[i] + S.cq32[i];
+      b += f32 ? S.cq32[i] : 0;
+    }
+    if (n64) *n64 = a;
+    if (n32) *n32 = b;
     return QX_OK;
   });
+}
+
+int qx_set_fp32_classes(qx_ctx* c, int pass, const int* flags) {
+  if (!c || pass < 0 || pass > 1) return QX_EINVAL;
+  for (int k = 0; k < QX_NUM_CLASSES; ++k) c->fp32_ok[pass][k] = flags ? (flags[k] != 0) : qx_fp32_default[k][pass];
+  return QX_OK;
+}
+
+int qx_get_fp32_classes(const qx_ctx* c, int pass, int* flags) {
+  if (!c || !flags || pass < 0 || pass > 1) return QX_EINVAL;
+  for (int k = 0; k < QX_NUM_CLASSES; ++k) flags[k] = c->fp32_ok[pass][k];
+  return QX_OK;
 }
 
 int qx_set_streams(qx_ctx* c, int n) {

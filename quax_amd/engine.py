@@ -123,12 +123,31 @@ class EriEngine:
             out.append((tuple(l4), eri[k], deri[k], npart.value))
         return out
 
-    def mixed_quartets(self, fp32_threshold: float):
-        """(FP64, FP32) contracted quartet counts of the mixed-precision split."""
+    def mixed_quartets(self, fp32_threshold: float, pass_=-1):
+        """(FP64, FP32) contracted quartet counts of a mixed-precision pass at
+        the threshold: pass_ 0 = J/K, 1 = gradient (the classes whose FP32
+        policy admits FP32), -1 = every class below the threshold."""
         n64, n32 = ctypes.c_longlong(), ctypes.c_longlong()
-        check(self.L.qx_mixed_quartets(self.ctx, float(fp32_threshold), ctypes.byref(n64), ctypes.byref(n32)),
-              self.ctx, "mixed_quartets")
+        check(self.L.qx_mixed_pass_quartets(self.ctx, float(fp32_threshold), int(pass_), ctypes.byref(n64),
+                                            ctypes.byref(n32)), self.ctx, "mixed_quartets")
         return n64.value, n32.value
+
+    def fp32_classes(self, pass_: int):
+        """Per kernel class, whether the mixed pass (0 J/K, 1 gradient) may use FP32."""
+        nc = self.L.qx_num_classes()
+        f = np.zeros(nc, dtype=np.int32)
+        check(self.L.qx_get_fp32_classes(self.ctx, int(pass_), f.ctypes.data_as(ctypes.c_void_p)), self.ctx)
+        return f.astype(bool)
+
+    def set_fp32_classes(self, pass_: int, flags=None):
+        """Sets the per-class FP32 policy of a mixed pass (None: built-in default)."""
+        if flags is None:
+            check(self.L.qx_set_fp32_classes(self.ctx, int(pass_), None), self.ctx)
+            return
+        f = np.ascontiguousarray(np.asarray(flags, dtype=np.int32))
+        if f.shape != (self.L.qx_num_classes(),):
+            raise ValueError("one flag per kernel class")
+        check(self.L.qx_set_fp32_classes(self.ctx, int(pass_), f.ctypes.data_as(ctypes.c_void_p)), self.ctx)
 
     def set_streams(self, n: int):
         check(self.L.qx_set_streams(self.ctx, int(n)), self.ctx, "set_streams")

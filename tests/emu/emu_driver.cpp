@@ -78,8 +78,12 @@ int main(int argc, char** argv) {
   if ((rc = qx_oei(c, S.data(), T.data(), V.data(), nullptr))) die(c, rc, "oei");
   if ((rc = qx_oei_grad(c, D.data(), D.data(), g1.data(), nullptr))) die(c, rc, "oei_grad");
   for (auto* v : {&J, &K, &g, &S, &T, &V, &g1}) std::fwrite(v->data(), 8, v->size(), o);
-  // mixed precision: every quartet in FP32 (threshold above any bound), then
-  // a split at 0.05 (both kernel variants in one pass)
+  // mixed precision, FP32 admitted in every class: every quartet in FP32
+  // (threshold above any bound), then a split at 0.05 (both masked launches
+  // over each list)
+  std::vector<int> all32(qx_num_classes(), 1);
+  for (int pass = 0; pass < 2; ++pass)
+    if ((rc = qx_set_fp32_classes(c, pass, all32.data()))) die(c, rc, "set_fp32_classes");
   for (double thr : {1e300, 0.05}) {
     std::vector<double> Jm(n2), Km(n2), gm(3 * natom, 0.0);
     if ((rc = qx_jk_build(c, D.data(), Jm.data(), Km.data(), 1, thr, nullptr))) die(c, rc, "jk_build mixed");

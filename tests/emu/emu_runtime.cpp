@@ -103,6 +103,16 @@ int shfl_xor(int v, int off) {
   return r;
 }
 
+int syncthreads_or(int p) {
+  const unsigned l = threadIdx.x;
+  g_pool->islot[l] = p ? 1 : 0;
+  g_sync->arrive_and_wait();
+  int r = 0;
+  for (unsigned i = 0; i < g_sync_n; ++i) r |= g_pool->islot[i];
+  g_sync->arrive_and_wait();
+  return r;
+}
+
 unsigned long long ballot(bool p) {
   const unsigned l = threadIdx.x;
   g_pool->islot[l] = p ? 1 : 0;

@@ -103,9 +103,11 @@ double shfl_xor(double v, int off);
 int shfl_xor(int v, int off);
 unsigned long long ballot(bool p);
 void syncthreads();
+int syncthreads_or(int p);
 }  // namespace emu
 
 #define __syncthreads() emu::syncthreads()
+inline int __syncthreads_or(int p) { return emu::syncthreads_or(p); }
 inline double __shfl_xor(double v, int off, int = 64) { return emu::shfl_xor(v, off); }
 inline int __shfl_xor(int v, int off, int = 64) { return emu::shfl_xor(v, off); }
 inline unsigned long long __ballot(bool p) { return emu::ballot(p); }

@@ -2,6 +2,9 @@
 
 Quartets whose Schwarz bound sqrt(Q_ab Q_cd) is below the threshold run the
 recursion in FP32 and are summed in FP64; the rest are the FP64 kernels.
+Each class list is evaluated by two launches masked to its FP64 and FP32
+kets, in the classes the per-class FP32 policy admits; the tests that check
+the FP32 kernels admit every class.
 The FP64 GPU path is the reference here: it is checked against the oracle
 by the other GPU tests.  An FP32 integral carries a relative error of a few
 FP32 ulps (~1e-7 of the quartet's magnitude, which the bound caps), so the
@@ -32,6 +35,12 @@ def _density(n, seed=1):
     return X @ X.T / n
 
 
+def _all_fp32(eng):
+    n = len(eng.fp32_classes(0))
+    for p in (0, 1):
+        eng.set_fp32_classes(p, np.ones(n, dtype=bool))
+
+
 def _c2():
     m = workloads.cluster10()
     return Molecule([m.symbol(i) for i in range(3)], m.geometry()[:3])
@@ -50,12 +59,36 @@ def test_split_counts_partition_the_quartets(gpu, maker):
             assert n64 + n32 == nq
             assert n32 >= last32          # a higher threshold moves more quartets to FP32
             last32 = n32
+            # a pass counts FP32 quartets in the classes its policy admits only
+            for p in (0, 1):
+                a, b = eng.mixed_quartets(thr, p)
+                assert a + b == nq and b <= n32
+
+
+def test_fp32_class_policy_round_trip(gpu):
+    mol, bs = small_d_cluster()
+    with _engine(mol, bs) as eng:
+        nq = eng.num_quartets()
+        default = [eng.fp32_classes(p) for p in (0, 1)]
+        # the built-in policy admits FP32 only in some (the heaviest) classes
+        assert all(0 < d.sum() < len(d) for d in default)
+        _all_fp32(eng)
+        assert all(eng.fp32_classes(p).all() for p in (0, 1))
+        assert eng.mixed_quartets(1e300, 0) == (0, nq)
+        eng.set_fp32_classes(1, np.zeros(len(default[1]), dtype=bool))
+        assert eng.mixed_quartets(1e300, 1) == (nq, 0)
+        for p in (0, 1):
+            eng.set_fp32_classes(p, None)
+            assert (eng.fp32_classes(p) == default[p]).all()
+        with pytest.raises(ValueError):
+            eng.set_fp32_classes(0, [1, 0])
 
 
 @pytest.mark.parametrize("maker", [small_d_cluster, contracted_d_cluster])
 def test_mixed_jk_and_gradient_vs_fp64(gpu, maker):
     mol, bs = maker()
     with _engine(mol, bs) as eng:
+        _all_fp32(eng)
         D = _density(eng.nbf, 3)
         J, K = (x.cpu().numpy() for x in eng.jk(D))
         g = eng.eri_grad(D).cpu().numpy()
@@ -74,6 +107,23 @@ def test_mixed_jk_and_gradient_vs_fp64(gpu, maker):
         assert np.abs(Jm - J).max() < 1e-9
         assert np.abs(Km - K).max() < 1e-9
         assert np.abs(gm - g).max() < 1e-9
+        # a threshold that splits most bra rows between the two masked
+        # launches: a quartet lost or counted twice would move J, K or the
+        # gradient by the size of an integral
+        thr = next(t for t in np.logspace(-4, 1, 16)
+                   if all(min(eng.mixed_quartets(t, p)) > 0 for p in (0, 1)))
+        Js, Ks = (x.cpu().numpy() for x in eng.jk(D, precision=MIXED, fp32_threshold=thr))
+        gs = eng.eri_grad(D, precision=MIXED, fp32_threshold=thr).cpu().numpy()
+        assert np.abs(Js - J).max() < 1e-5
+        assert np.abs(Ks - K).max() < 1e-5
+        assert np.abs(gs - g).max() < 1e-5
+        # the built-in policy: FP64 kernels in the classes it does not admit
+        for p in (0, 1):
+            eng.set_fp32_classes(p, None)
+        Jd, Kd = (x.cpu().numpy() for x in eng.jk(D, precision=MIXED, fp32_threshold=1e300))
+        gd = eng.eri_grad(D, precision=MIXED, fp32_threshold=1e300).cpu().numpy()
+        assert np.abs(Jd - J).max() < 1e-5 and np.abs(Kd - K).max() < 1e-5
+        assert np.abs(gd - g).max() < 1e-5
 
 
 def test_mixed_fp32_matches_oracle_loosely(gpu):
@@ -83,6 +133,7 @@ def test_mixed_fp32_matches_oracle_loosely(gpu):
     G = ob.compute_2e_int(1e-14)
     D = _density(ob.nbf, 5)
     with _engine(mol, bs, tol=1e-14) as eng:
+        _all_fp32(eng)
         J, K = (x.cpu().numpy() for x in eng.jk(D, precision=MIXED, fp32_threshold=1e300))
     assert np.abs(J - np.einsum("pqrs,rs->pq", G, D)).max() < 1e-5
     assert np.abs(K - np.einsum("prqs,rs->pq", G, D)).max() < 1e-5
