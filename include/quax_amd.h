@@ -224,6 +224,48 @@ int qx_mixed_pass_quartets(qx_ctx* ctx, double fp32_threshold, int pass, long lo
 int qx_set_fp32_classes(qx_ctx* ctx, int pass, const int* flags);
 int qx_get_fp32_classes(const qx_ctx* ctx, int pass, int* flags);
 
+/* RHF SCF iterations captured into a HIP graph (replaces the SCF loop of
+ * quax/methods/hartree_fock.py:83-99, a jax.lax.while_loop: J/K, Fock
+ * matrix, energy, DIIS error A (F D S - S D F) A, density of the
+ * orthogonalised Fock matrix), for one molecule or a batch of molecules
+ * that are fragments of ctx.  Matrices are nmol x nm x nm device arrays,
+ * molecule b in the leading nbf[b] x nbf[b] block of its slice (zero
+ * elsewhere); offset[b] is its first AO in ctx.  H = T + V, S, A =
+ * inv(chol(S))^T per molecule; nocc: doubly occupied orbitals; enuc: host
+ * nuclear repulsion energies.  The density comes from trace-correcting
+ * purification (at most purify_iters steps of X <- X^2 or 2X - X^2, the
+ * rest skipped once tr(X - X^2) < 1e-8, then two McWeeny steps
+ * X <- 3X^2 - 2X^3; FP64 MFMA GEMMs) instead of an eigensolver, so an
+ * iteration has no host synchronisation.
+ * diis: Pulay DIIS on the error above with diis_space (2..32) vectors;
+ * spectral_shift: + diag(linspace(0, 1, n)) 1e-9 on the orthogonalised Fock
+ * matrix (hartree_fock.py:38-40); convergence |dE| <= conv and RMS error
+ * <= conv, or maxit iterations.  precision / fp32_threshold: the J/K pass.
+ *   qx_scf_init: the first step from D0 (eager; allocates what capture needs).
+ *   qx_scf_step: n iterations, replays of one captured graph when use_graph
+ *     (captured on first use; a molecule that has converged is skipped by
+ *     every dense kernel), else the same kernels launched directly.
+ *   qx_scf_status: synchronises the stream; per molecule E, dE, RMS error,
+ *     iterations, done flag and the idempotency defect tr(X - X^2) before
+ *     the last McWeeny step (host arrays, any may be NULL).
+ *   qx_scf_matrix: 0 = D, 1 = D_old (the density that built F), 2 = F (as
+ *     used for D, DIIS-extrapolated with DIIS), 3 = F before extrapolation.
+ * Graph replays and other passes of ctx must run on one stream (or the
+ * caller synchronises between them).  Errors: qx_scf_last_error. */
+typedef struct qx_scf qx_scf;
+int qx_scf_create(qx_ctx* ctx, int nmol, int nm, const int* nbf, const int* offset, const int* nocc,
+                  const double* enuc, const double* H, const double* S, const double* A, int diis, int diis_space,
+                  int spectral_shift, double conv, int maxit, int purify_iters, int precision,
+                  double fp32_threshold, qx_scf** out);
+int qx_scf_init(qx_scf* scf, const double* D0, void* stream);
+int qx_scf_capture(qx_scf* scf, void* stream);
+int qx_scf_step(qx_scf* scf, int n, int use_graph, void* stream);
+int qx_scf_status(qx_scf* scf, double* energy, double* de, double* drms, int* iterations, int* done,
+                  double* idempotency, void* stream);
+int qx_scf_matrix(qx_scf* scf, int which, double* out, void* stream);
+void qx_scf_destroy(qx_scf* scf);
+const char* qx_scf_last_error(const qx_scf* scf);
+
 /* Independent class kernels of a J/K or gradient pass are spread over
  * nstreams streams (1..8; default 4): the caller's stream plus side streams
  * of default priority that join back into it before the call returns. */

@@ -76,6 +76,15 @@ SIGNATURES = {
     "qx_set_fp32_classes": (_i, [_vp, _i, _vp]),
     "qx_get_fp32_classes": (_i, [_vp, _i, _vp]),
     "qx_set_streams": (_i, [_vp, _i]),
+    "qx_scf_create": (_i, [_vp, _i, _i, _ip, _ip, _ip, _dp, _vp, _vp, _vp, _i, _i, _i, ctypes.c_double, _i, _i, _i,
+                           ctypes.c_double, ctypes.POINTER(_vp)]),
+    "qx_scf_init": (_i, [_vp, _vp, _vp]),
+    "qx_scf_capture": (_i, [_vp, _vp]),
+    "qx_scf_step": (_i, [_vp, _i, _i, _vp]),
+    "qx_scf_status": (_i, [_vp, _dp, _dp, _dp, _ip, _ip, _dp, _vp]),
+    "qx_scf_matrix": (_i, [_vp, _i, _vp, _vp]),
+    "qx_scf_destroy": (None, [_vp]),
+    "qx_scf_last_error": (ctypes.c_char_p, [_vp]),
     "qx_class_info": (_i, [_i, _ip, _ip]),
     "qx_class_quartets": (_i, [_vp, ctypes.POINTER(_ll), ctypes.POINTER(_ll)]),
     "qx_class_flops": (_i, [_i, _dp, _dp, _ip]),

@@ -33,7 +33,7 @@ FLAGS = ["-std=c++17", "-O3", f"--offload-arch={ARCH}", "-fPIC", "-munsafe-fp-at
          "-Wno-unused-variable", "-Wno-unused-but-set-variable",
          f"-I{os.path.join(ROOT, 'include')}", f"-I{HERE}"]
 
-RUNTIME_SOURCES = ["qx_runtime.hip", "qx_kernels.hip", "qx_gemm.hip"]
+RUNTIME_SOURCES = ["qx_runtime.hip", "qx_kernels.hip", "qx_gemm.hip", "qx_scf.hip"]
 HEADERS = ["qx_device.h", "qx_runtime.h"]
 
 

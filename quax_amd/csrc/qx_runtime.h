@@ -110,6 +110,10 @@ struct qx_ctx {
   double* scratch = nullptr;
   hipEvent_t ws_ev = nullptr;
   bool ws_used = false;
+  // set while a caller captures a J/K pass into a HIP graph: the pass then
+  // records and waits on no event outside the capture (the graph's own
+  // stream order sequences its passes)
+  bool capturing = false;
 
   // mixed precision: per class and pass (0 J/K, 1 gradient) whether quartets
   // below the threshold run in FP32 (classes whose FP32 kernels are faster)

@@ -518,11 +518,12 @@ double* begin_ws(qx_ctx* c, hipStream_t s) {
     c->ws = dalloc<double>(c, (size_t)5 * c->ncart * c->ncart);
     QX_HIP(hipEventCreateWithFlags(&c->ws_ev, hipEventDisableTiming));
   }
-  if (c->ws_used) QX_HIP(hipStreamWaitEvent(s, c->ws_ev, 0));
+  if (c->ws_used && !c->capturing) QX_HIP(hipStreamWaitEvent(s, c->ws_ev, 0));
   return c->ws;
 }
 
 void end_ws(qx_ctx* c, hipStream_t s) {
+  if (c->capturing) return;
   QX_HIP(hipEventRecord(c->ws_ev, s));
   c->ws_used = true;
 }

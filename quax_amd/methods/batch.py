@@ -26,9 +26,16 @@ _PAD = 1.0e4  # diagonal of the padding rows: above every occupied orbital energ
 
 
 def batched_rhf(molecules, basis_name="even-tempered", pure=True, ints_tol=1e-12, conv=1e-10, maxit=200,
-                diis_space=20, precision="fp64", fp32_threshold=1e-8, gradient=True, device=0):
+                diis_space=20, precision="fp64", fp32_threshold=1e-8, gradient=True, device=0, graph=False,
+                purify_iters=100):
     """Returns (energies, gradients or None, aux) for a list of neutral
-    closed-shell molecules; aux holds iterations per molecule."""
+    closed-shell molecules; aux holds iterations per molecule.  graph: the
+    iterations of all molecules replayed from one captured HIP graph
+    (GraphSCF: the single-molecule iteration of restricted_hartree_fock per
+    molecule, DIIS on, no spectral shift, densities by purification on the
+    device, converged molecules skipped; a molecule whose DIIS error stalls at
+    the purification's rounding floor finishes in the host-driven loop);
+    otherwise the host-driven lock-step loop (_eager_scf)."""
     import torch
     from ..engine import EriEngine
     dev = torch.device("cuda", device)
@@ -63,13 +70,47 @@ def batched_rhf(molecules, basis_name="even-tempered", pure=True, ints_tol=1e-12
         Z = [m.charge(i) for i in range(m.natom())]
         Enuc.append(nuclear_repulsion(m.geometry(), Z))
     blk = [slice(int(off[k]), int(off[k + 1])) for k in range(B)]
+    if graph:
+        E, D, C_all, eps_all, iters, done, stalled = _graph_scf(eng, H, S, A, ndocc, Enuc, off, blk, ntot, conv,
+                                                                maxit, diis_space, prec, fp32_threshold,
+                                                                purify_iters, dev)
+        if stalled.any():
+            # the last iterations of molecules whose purified densities hit
+            # their rounding floor: eigh-based, from the graph's densities
+            E2, D2, C2, eps2, it2, done2 = _eager_scf(eng, H, S, A, ndocc, Enuc, blk, nb, ntot, conv, maxit,
+                                                      diis_space, prec, fp32_threshold, dev,
+                                                      D0=[D[b_, b_] for b_ in blk], active=stalled)
+            for k in np.flatnonzero(stalled):
+                E[k], C_all[k], eps_all[k] = E2[k], C2[k], eps2[k]
+                D[blk[k], blk[k]] = D2[blk[k], blk[k]]
+                iters[k] += it2[k]
+                done[k] = done2[k]
+        return _finish(eng, one, molecules, D, C_all, eps_all, blk, aof, ndocc, E, iters, done, nb, gradient, prec,
+                       fp32_threshold)
+    E, D, C_all, eps_all, iters, done = _eager_scf(eng, H, S, A, ndocc, Enuc, blk, nb, ntot, conv, maxit, diis_space,
+                                                   prec, fp32_threshold, dev)
+    return _finish(eng, one, molecules, D, C_all, eps_all, blk, aof, ndocc, E, iters, done, nb, gradient, prec,
+                   fp32_threshold)
+
+
+def _eager_scf(eng, H, S, A, ndocc, Enuc, blk, nb, ntot, conv, maxit, diis_space, prec, fp32_threshold, dev,
+               D0=None, active=None):
+    """The host-driven lock-step loop over the molecules flagged in `active`
+    (all by default), each from its block of D0 (zero by default); the other
+    molecules' blocks stay as D0 gives them."""
+    import torch
+    dt = torch.float64
+    B, nmax = len(H), max(nb)
     D = torch.zeros((ntot, ntot), dtype=dt, device=dev)
+    if D0 is not None:
+        for k in range(B):
+            D[blk[k], blk[k]] = D0[k]
     Fp = torch.zeros((B, nmax, nmax), dtype=dt, device=dev)
     pad = torch.diag(_PAD + torch.arange(nmax, dtype=dt, device=dev))
     diis_F = [[] for _ in range(B)]
     diis_e = [[] for _ in range(B)]
     E = np.full(B, np.inf)
-    done = np.zeros(B, dtype=bool)
+    done = np.zeros(B, dtype=bool) if active is None else ~np.asarray(active, dtype=bool)
     iters = np.zeros(B, dtype=int)
     C_all, eps_all = [None] * B, [None] * B
 
@@ -133,6 +174,39 @@ def batched_rhf(molecules, basis_name="even-tempered", pure=True, ints_tol=1e-12
             D[b_, b_] = Cocc @ Cocc.T
         if done.all():
             break
+    return E, D, C_all, eps_all, iters, done
+
+
+def _graph_scf(eng, H, S, A, ndocc, Enuc, off, blk, ntot, conv, maxit, diis_space, prec, thr, purify_iters, dev):
+    import torch
+    from .scf_graph import SCF_D, SCF_F, GraphSCF
+    B = len(H)
+    scf = GraphSCF(eng, H, S, A, ndocc, Enuc, offsets=[int(o) for o in off[:B]], diis=True, diis_space=diis_space,
+                   spectral_shift=False, conv=conv, maxit=maxit, purify_iters=purify_iters, precision=prec,
+                   fp32_threshold=thr)
+    try:
+        scf.init([torch.zeros_like(h) for h in H])
+        st = scf.run(use_graph=True)
+        stalled = st["stalled"]
+        bad = [k for k in range(B) if not abs(st["idempotency"][k]) < 1e-8]
+        if bad:
+            raise RuntimeError(f"density purification did not converge for molecules {bad}; raise purify_iters")
+        Ds, Fs = scf.matrix(SCF_D), scf.matrix(SCF_F)
+    finally:
+        scf.close()
+    D = torch.zeros((ntot, ntot), dtype=torch.float64, device=dev)
+    C_all, eps_all = [], []
+    for k in range(B):
+        D[blk[k], blk[k]] = Ds[k]
+        w_, v_ = np.linalg.eigh((A[k].T @ Fs[k] @ A[k]).cpu().numpy())
+        C_all.append(A[k] @ torch.as_tensor(v_, device=dev))
+        eps_all.append(torch.as_tensor(w_, device=dev))
+    converged = (np.abs(st["dE"]) <= conv) & (st["drms"] <= conv)
+    return list(st["E"]), D, C_all, eps_all, np.asarray(st["iterations"]), converged, stalled
+
+
+def _finish(eng, one, molecules, D, C_all, eps_all, blk, aof, ndocc, E, iters, done, nb, gradient, prec,
+            fp32_threshold):
     grads = None
     if gradient:
         g2 = eng.eri_grad(D, precision=prec, fp32_threshold=fp32_threshold).reshape(-1, 3)

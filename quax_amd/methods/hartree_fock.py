@@ -38,6 +38,8 @@ DEFAULT_OPTIONS = {
     "eigh": "auto",               # Fock diagonalisation: "device", "host" or "auto" (host up to 512)
     "pure": False,
     "device": 0,
+    "graph": False,               # iterations replayed from one captured HIP graph (qx_scf_*)
+    "purify_iters": 100,          # graph SCF: most purification steps per density
 }
 
 
@@ -125,6 +127,19 @@ def restricted_hartree_fock(engine, nelectrons, options=None, return_aux_data=Fa
     conv = opts["conv"] if opts["conv"] is not None else (1e-9 if opts["spectral_shift"] else 1e-10)
     prec = QX_PREC_MIXED if opts["precision"] == "mixed" else QX_PREC_FP64
     thr = float(opts["fp32_threshold"])
+
+    if opts["graph"]:
+        if jk is not None or opts["damping"]:
+            raise ValueError("the graph SCF has no custom J/K and no damping")
+        from .scf_graph import rhf_graph
+        if D0 is not None:
+            Dg = torch.as_tensor(D0, dtype=dt).to(dev)
+            if tuple(Dg.shape) != tuple(H.shape):
+                raise ValueError("D0 must be nbf x nbf")
+        else:
+            Dg = H.clone() if opts["guess_core"] else torch.zeros_like(H)
+        E, aux = rhf_graph(engine, nelectrons, opts, H, S, A, Enuc, Dg)
+        return (E, aux) if return_aux_data else E
 
     def build_jk(D):
         if jk is not None:
