@@ -236,12 +236,13 @@ def cpu_baseline(mol, basis, tol, eng, seconds):
     return res, nthreads
 
 
-def hf_energy_gradient(eng, mol, precision, thr):
-    """RHF energy (DIIS, conv 1e-10) + analytic gradient; wall times."""
+def hf_energy_gradient(eng, mol, precision, thr, graph=False):
+    """RHF energy (DIIS, conv 1e-10) + analytic gradient; wall times.
+    graph: SCF iterations replayed from one captured HIP graph."""
     import torch
     from quax_amd.methods.hartree_fock import restricted_hartree_fock, rhf_gradient
     opts = {"diis": True, "spectral_shift": False, "conv": 1e-10, "maxit": 200, "precision": precision,
-            "fp32_threshold": thr}
+            "fp32_threshold": thr, "graph": graph}
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     E, aux = restricted_hartree_fock(eng, mol.nelectron(), opts, return_aux_data=True)
@@ -312,9 +313,11 @@ def timed_steps(eng, D, steps, warmup, precision, thr, profile):
 
 def bench_batch(args):
     """Batched RHF energy + analytic gradient of the first --batch-size
-    molecules of the seeded 256-molecule set (3-12 atoms, C/H/O), FP64, one
-    integral context with the molecules as fragments; the first three are
-    checked against single-molecule runs."""
+    molecules of the seeded 256-molecule set (3-12 atoms, C/H/O), FP64: each
+    chunk of --chunk molecules is one integral context with the molecules as
+    fragments, its SCF iterations replayed from one captured HIP graph
+    (converged molecules masked out of J/K), then one adjoint-gradient pass;
+    the first three molecules are checked against single-molecule runs."""
     import numpy as np
     import torch
     from quax_amd import workloads
@@ -323,10 +326,20 @@ def bench_batch(args):
     from quax_amd.methods.batch import batched_rhf
     from quax_amd.methods.hartree_fock import restricted_hartree_fock, rhf_gradient
     mols = workloads.batch256()[:args.batch_size]
-    batched_rhf(mols[:2], "even-tempered", pure=True, ints_tol=1e-12, conv=1e-6)   # warm-up
+    graph = not args.no_graph
+    batched_rhf(mols[:2], "even-tempered", pure=True, ints_tol=1e-12, conv=1e-6, graph=graph)   # warm-up
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    E, G, aux = batched_rhf(mols, "even-tempered", pure=True, ints_tol=1e-12, conv=1e-10)
+    E, G, aux = [], [], {"iterations": [], "converged": [], "nbf": []}
+    for c0 in range(0, len(mols), args.chunk):
+        Ec, Gc, ac = batched_rhf(mols[c0:c0 + args.chunk], "even-tempered", pure=True, ints_tol=1e-12, conv=1e-10,
+                                 graph=graph)
+        E += Ec
+        G += Gc
+        for k in aux:
+            aux[k] += ac[k]
+        log(f"[bench] batch molecules {c0}-{c0 + len(Ec) - 1}: {time.perf_counter() - t0:.1f} s so far, "
+            f"iterations max {max(ac['iterations'])}")
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     errs = []
@@ -346,6 +359,7 @@ def bench_batch(args):
            "config": {"workload": f"first {len(mols)} of 256 seeded molecules (3-12 atoms, C/H/O 0.4/0.4/0.2), "
                                   f"even-tempered spherical basis, {sum(aux['nbf'])} AO total, Schwarz 1e-12, "
                                   "conv 1e-10",
+                      "chunk": args.chunk, "graph_scf": graph,
                       "wall_s": wall, "iterations_max": max(aux["iterations"]),
                       "all_converged": all(aux["converged"]),
                       "single_molecule_s_each": single,
@@ -362,7 +376,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU baseline budget (0 = skip)")
     ap.add_argument("--workload", default="cluster10", choices=["cluster10", "cluster24", "water", "batch"])
     ap.add_argument("--batch-size", type=int, default=32,
-                    help="batch workload: molecules of the seeded 256-molecule set per batched run")
+                    help="batch workload: molecules of the seeded 256-molecule set")
+    ap.add_argument("--chunk", type=int, default=32, help="batch workload: molecules per context and graph")
+    ap.add_argument("--no-graph", action="store_true", help="batch workload: host-driven lock-step SCF")
     ap.add_argument("--fp32-threshold", type=float, default=1e-8,
                     help="Schwarz bound below which the mixed runs use FP32 quartets")
     ap.add_argument("--streams", type=int, default=4, help="streams per pass (1 = every kernel timed alone)")
@@ -447,6 +463,14 @@ def main():
         Em, auxm, gradm, t_scfm, t_gradm = hf_energy_gradient(eng, mol, "mixed", thr)
         log(f"[bench] mixed RHF E = {Em:.10f} (error {Em - E:.2e}) in {auxm['iterations']} iterations "
             f"({t_scfm:.2f} s), gradient {t_gradm:.3f} s, max gradient error {np.abs(gradm - grad).max():.2e}")
+        Eg, auxg, gradg, t_scfg, t_gradg = hf_energy_gradient(eng, mol, "fp64", thr, graph=True)
+        log(f"[bench] graph-SCF RHF E = {Eg:.10f} (vs eager {Eg - E:.2e}) in {auxg['iterations']} iterations "
+            f"({auxg.get('graph_iterations', auxg['iterations'])} replayed; {t_scfg:.2f} s), "
+            f"max gradient difference {np.abs(gradg - grad).max():.2e}")
+        hf.update({"scf_graph_wall_s": t_scfg, "scf_graph_iterations": auxg["iterations"],
+                   "scf_graph_replays": auxg.get("graph_iterations", auxg["iterations"]),
+                   "scf_graph_energy_vs_eager": Eg - E,
+                   "scf_graph_grad_vs_eager": float(np.abs(gradg - grad).max())})
         D = aux["D"].contiguous()
         sweep = []
         for t in (float(x) for x in args.sweep.split(",") if x.strip()):

@@ -266,6 +266,24 @@ int qx_scf_matrix(qx_scf* scf, int which, double* out, void* stream);
 void qx_scf_destroy(qx_scf* scf);
 const char* qx_scf_last_error(const qx_scf* scf);
 
+/* Quartet tiles (no reference counterpart): threads per workgroup -- the
+ * primitive quartets one workgroup evaluates, and the quartets that share
+ * its LDS K' rows -- of the streaming kernels of each class, per pass
+ * (0 = J/K, 1 = adjoint gradient); block[qx_num_classes()], multiples of
+ * 64 up to 256, or NULL for the built-in table (picked by the autotuner,
+ * quax_amd/csrc/tune.py --tiles, from GPU timings). */
+int qx_set_class_tiles(qx_ctx* ctx, int pass, const int* block);
+int qx_get_class_tiles(const qx_ctx* ctx, int pass, int* block);
+
+/* qx_jk_build for a batch context (qx_initialize_fragments) that leaves out
+ * the quartets of every fragment f with frag_skip[f] != 0 (device int array,
+ * one entry per fragment id): their blocks of J and K come out zero.  Blocks
+ * of quartets that are all left out exit at once, so a batch whose
+ * molecules converge at different iterations stops paying for the
+ * converged ones (no reference counterpart). */
+int qx_jk_build_masked(qx_ctx* ctx, const double* D, double* J, double* K, int precision, double fp32_threshold,
+                       const int* frag_skip, void* stream);
+
 /* Independent class kernels of a J/K or gradient pass are spread over
  * nstreams streams (1..8; default 4): the caller's stream plus side streams
  * of default priority that join back into it before the call returns. */

@@ -76,6 +76,9 @@ SIGNATURES = {
     "qx_set_fp32_classes": (_i, [_vp, _i, _vp]),
     "qx_get_fp32_classes": (_i, [_vp, _i, _vp]),
     "qx_set_streams": (_i, [_vp, _i]),
+    "qx_set_class_tiles": (_i, [_vp, _i, _vp]),
+    "qx_get_class_tiles": (_i, [_vp, _i, _vp]),
+    "qx_jk_build_masked": (_i, [_vp, _vp, _vp, _vp, _i, ctypes.c_double, _vp, _vp]),
     "qx_scf_create": (_i, [_vp, _i, _i, _ip, _ip, _ip, _dp, _vp, _vp, _vp, _i, _i, _i, ctypes.c_double, _i, _i, _i,
                            ctypes.c_double, ctypes.POINTER(_vp)]),
     "qx_scf_init": (_i, [_vp, _vp, _vp]),

@@ -47,6 +47,14 @@ def _hash(paths, extra):
 
 
 def _flags(src):
+    """Compiler flags of one translation unit: the runtime sources also see
+    the generated class tables (qx_classes.inc in GEN)."""
+    if not src.startswith(GEN):
+        return _opt_flags(src) + [f"-I{GEN}"]
+    return _opt_flags(src)
+
+
+def _opt_flags(src):
     """Tangent (second-derivative) kernels build at -O2: at -O3 the gfx950
     compiler returned a wrong value for one element of the inlined (dd|ps)
     tangent part (cls_2210_t3: (d_yy d_zz | p_j s), checked against the same
@@ -59,8 +67,9 @@ def _flags(src):
 
 def _compile(src, force):
     if src.startswith(GEN):
-        # generated class kernels include only qx_device.h
-        deps = [src, os.path.join(HERE, "qx_device.h"), os.path.join(GEN, "qx_classes.inc")]
+        # generated class kernels include only qx_device.h (the class tables
+        # in qx_classes.inc are compiled into qx_runtime.hip alone)
+        deps = [src, os.path.join(HERE, "qx_device.h")]
     else:
         deps = [src] + [os.path.join(HERE, h) for h in HEADERS] + [os.path.join(ROOT, "include", "quax_amd.h"),
                                                                    os.path.join(GEN, "qx_classes.inc")]
@@ -76,6 +85,31 @@ def _compile(src, force):
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr[-4000:]}")
     os.replace(obj + ".tmp", obj)
     return obj, time.time() - t0, False
+
+
+H5_PREFIX = os.environ.get("QX_HDF5_PREFIX", "/opt/conda")
+H5_LIB = os.path.join(LIBDIR, "libquax_h5.so")
+
+
+def build_h5(verbose=True):
+    """libquax_h5.so (HDF5 files of the disk mode): host C on the HDF5 C
+    library under QX_HDF5_PREFIX (default /opt/conda); skipped with a message
+    when that library is absent."""
+    inc, lib = os.path.join(H5_PREFIX, "include"), os.path.join(H5_PREFIX, "lib")
+    if not (os.path.exists(os.path.join(inc, "hdf5.h")) and os.path.exists(os.path.join(lib, "libhdf5.so"))):
+        if verbose:
+            print(f"  HDF5 not found under {H5_PREFIX}: libquax_h5.so (disk mode) not built", flush=True)
+        return None
+    os.makedirs(LIBDIR, exist_ok=True)
+    cmd = ["gcc", "-O2", "-shared", "-fPIC", "-Wall", f"-I{os.path.join(ROOT, 'include')}", f"-I{inc}",
+           os.path.join(HERE, "qx_h5.c"), f"-L{lib}", "-lhdf5", f"-Wl,-rpath,{lib}", "-o", H5_LIB + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"gcc failed for qx_h5.c:\n{r.stderr[-4000:]}")
+    os.replace(H5_LIB + ".tmp", H5_LIB)
+    if verbose:
+        print(f"built {H5_LIB}", flush=True)
+    return H5_LIB
 
 
 def build(force=False, jobs=None, lmax=2, fp32=True, verbose=True, budget=None, out=None, waves=None,
@@ -119,6 +153,8 @@ def build(force=False, jobs=None, lmax=2, fp32=True, verbose=True, budget=None, 
     os.replace(lib + ".tmp", lib)
     if verbose:
         print(f"built {lib} ({len(objs)} objects, {time.time() - t0:.1f} s)", flush=True)
+    if out is None:
+        build_h5(verbose)
     return lib
 
 

@@ -37,11 +37,13 @@ AX = "xyz"
 # a per-class table (picked by timing budget variants on the GPU), else
 # DERIV_NODES; QX_DERIV_NODES in the environment overrides both (tuning builds)
 DERIV_NODES = 2000
-_BUDGET_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "deriv_budget.json")
+# the tuning tables (QX_TUNING_FILE names another file: tests, tuning builds)
+_BUDGET_FILE = os.environ.get("QX_TUNING_FILE") or os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                                                                "deriv_budget.json")
 DERIV_BUDGET = {}
 if os.path.exists(_BUDGET_FILE):  # written by tune.py from GPU timings
     with open(_BUDGET_FILE) as _fh:
-        DERIV_BUDGET = {tuple(int(ch) for ch in k): int(v) for k, v in json.load(_fh)["budget"].items()}
+        DERIV_BUDGET = {tuple(int(ch) for ch in k): int(v) for k, v in json.load(_fh).get("budget", {}).items()}
 _BUDGET_OVERRIDE = os.environ.get("QX_DERIV_NODES")
 # waves per SIMD the adjoint kernels of a class are compiled for (0: the
 # compiler's choice, one wave with 512 registers for the heavy classes);
@@ -58,6 +60,14 @@ if os.path.exists(_BUDGET_FILE):
     with open(_BUDGET_FILE) as _fh:
         DERIV_CSPLIT = {tuple(int(ch) for ch in k): int(v) for k, v in json.load(_fh).get("csplit", {}).items()}
 _CSPLIT_OVERRIDE = os.environ.get("QX_CSPLIT")
+
+# threads per block of the streaming J/K and gradient kernels per class
+# (quartet tiles picked by tune.py --tiles from GPU timings; default 128)
+TILES = {"jk": {}, "grad": {}}
+if os.path.exists(_BUDGET_FILE):
+    with open(_BUDGET_FILE) as _fh:
+        _t = json.load(_fh).get("tiles", {})
+        TILES = {k: {tuple(int(ch) for ch in c): int(v) for c, v in _t.get(k, {}).items()} for k in ("jk", "grad")}
 
 # classes whose mixed-precision passes may use FP32 (J/K, gradient): those
 # whose FP32 kernels timed faster than their FP64 ones (fewer spills)
@@ -687,7 +697,7 @@ def emit_class_files(cls, outdir, fp32):
         w = deriv_waves(cls)
         attr = f" __attribute__((amdgpu_waves_per_eu({w}, {w})))" if w > 0 else ""
         kern = ["template <typename R>",
-                f"__global__ __launch_bounds__(QX_BLOCK){attr} void qx_kg_d{nm}_g{gi}(QxArgs A_) {{",
+                f"__global__ __launch_bounds__(QX_BLOCK_MAX){attr} void qx_kg_d{nm}_g{gi}(QxArgs A_) {{",
                 "  __shared__ double sg[QX_GRAD_LDS];",
                 "  qx_grad_body(A_, sg, [&](int P1, int P2, int u1, int u2, double (&g)[QX_GRAD_NCOMB]) {",
                 f"    qx_gr_d{nm}_{pi}<R>(A_, P1, P2, u1, u2, g);", "  });", "}",
@@ -752,6 +762,11 @@ def main():
     D.append("static const unsigned char qx_fp32_default[QX_NUM_CLASSES][2] = {")
     for cls in classes:
         D.append(f"  {{{int(cls in FP32_CLASSES['jk'])}, {int(cls in FP32_CLASSES['grad'])}}},")
+    D.append("};")
+    D.append("// per class: threads per block of the streaming J/K and gradient kernels")
+    D.append("static const int qx_tile_default[QX_NUM_CLASSES][2] = {")
+    for cls in classes:
+        D.append(f"  {{{TILES['jk'].get(cls, 128)}, {TILES['grad'].get(cls, 128)}}},")
     D.append("};")
     with open(os.path.join(outdir, "qx_classes.inc"), "w") as fh:
         fh.write("\n".join(D) + "\n")

@@ -15,7 +15,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#define QX_BLOCK 128
+#define QX_BLOCK 128      // threads per block of the array-form kernels; default of the streaming ones
+#define QX_BLOCK_MAX 256  // largest block of the streaming J/K and gradient kernels (a multiple of 64)
 #define QX_TWO_PI_5_2 34.986836655249725  // 2 pi^(5/2)
 #define QX_PI 3.14159265358979323846
 
@@ -89,6 +90,8 @@ struct QxEpiDev {
   const double* tv;          // 3N geometry direction: tangent kernels' seed, JKD weights
   const int* slot_map;       // DFULL: output slice of coordinate y (-1: none); null = coord rule
   int skip_single;           // FULL: leave out quartets with all four centres on one atom
+  const int* frag_skip;      // JK: per fragment, nonzero = leave its quartets out (null: none)
+  const int* pair_frag;      // JK: fragment of each sorted shell pair (with frag_skip)
 };
 
 struct QxArgs {
@@ -102,7 +105,10 @@ struct QxArgs {
 struct QxLaunch {
   int fp32;                  // recursion in FP32 (JK / GRAD epilogues only)
   hipStream_t stream;
+  int block;                 // JK / GRAD: threads per block (multiple of 64, <= QX_BLOCK_MAX; 0 = QX_BLOCK)
 };
+
+__host__ __device__ inline int qx_launch_block(const QxLaunch* L) { return L->block > 0 ? L->block : QX_BLOCK; }
 
 template <int M>
 __device__ __forceinline__ void qx_boys(double T, const double* __restrict__ tab, double (&F)[M + 1]) {
@@ -219,9 +225,10 @@ __device__ __forceinline__ bool qx_locate(const QxArgs& A_, long long t, int& u1
 }
 
 // true when no thread of the block holds a quartet of this launch's
-// precision (block-uniform; every thread must call it)
+// precision and of a fragment not left out (block-uniform; every thread must
+// call it)
 __device__ __forceinline__ bool qx_block_idle(const QxArgs& A_, bool active) {
-  return A_.w.sel != 0 && !__syncthreads_or(active ? 1 : 0);
+  return (A_.w.sel != 0 || A_.e.frag_skip) && !__syncthreads_or(active ? 1 : 0);
 }
 
 __device__ __forceinline__ void qx_atomic_add(double* p, double v) { unsafeAtomicAdd(p, v); }
@@ -605,20 +612,22 @@ __device__ __forceinline__ void qx_epi_djk(
 // operations and barriers run in converged, block-uniform control flow.
 #define QX_DEFINE_JK_KERNEL(NAME, NA, NB, NC, ND, WITH32, A0, A1)                                 \
   template <typename R>                                                                           \
-  __global__ __launch_bounds__(QX_BLOCK) void qx_kj_##NAME(QxArgs A_) {                           \
+  __global__ __launch_bounds__(QX_BLOCK_MAX) void qx_kj_##NAME(QxArgs A_) {                       \
     HIP_DYNAMIC_SHARED(double, qx_sk)                                                             \
-    const long long t = (long long)blockIdx.x * QX_BLOCK + threadIdx.x;                          \
+    const int bs_ = (int)blockDim.x;                                                              \
+    const long long t = (long long)blockIdx.x * bs_ + threadIdx.x;                                \
     bool active = t < A_.w.nquart;                                                                \
     int u1 = 0, u2 = 0, P1 = -1, P2 = -1;                                                         \
     if (active) active = qx_locate(A_, t, u1, u2, P1, P2);                                        \
+    if (active && A_.e.frag_skip && A_.e.frag_skip[A_.e.pair_frag[P1]]) active = false;           \
     if (qx_block_idle(A_, active)) return;                                                        \
     if (!active) P1 = P2 = -1;                                                                    \
     const int n_ = A_.e.nbf;                                                                      \
     double* sk = nullptr;                                                                         \
     int ra = 0, rb = 0;                                                                           \
     if (A_.e.krows) { /* same result in every thread: block-uniform */                            \
-      const long long t0 = (long long)blockIdx.x * QX_BLOCK;                                      \
-      const long long t1 = min(t0 + QX_BLOCK, A_.w.nquart) - 1;                                   \
+      const long long t0 = (long long)blockIdx.x * bs_;                                           \
+      const long long t1 = min(t0 + bs_, A_.w.nquart) - 1;                                        \
       int v1, v2, B0, B1, w2;                                                                     \
       qx_locate(A_, t0, v1, v2, B0, w2);                                                          \
       qx_locate(A_, t1, v1, v2, B1, w2);                                                          \
@@ -626,7 +635,7 @@ __device__ __forceinline__ void qx_epi_djk(
         sk = qx_sk;                                                                               \
         ra = A_.pr.bfa[B0];                                                                       \
         rb = A_.pr.bfb[B0];                                                                       \
-        for (int i = threadIdx.x; i < (NA + NB) * n_; i += QX_BLOCK) sk[i] = 0.0;                 \
+        for (int i = threadIdx.x; i < (NA + NB) * n_; i += bs_) sk[i] = 0.0;                      \
         __syncthreads();                                                                          \
       }                                                                                           \
     }                                                                                             \
@@ -636,7 +645,7 @@ __device__ __forceinline__ void qx_epi_djk(
     qx_jk_flush<NA, NB, NC, ND, A0, A1>(A_, P1, P2, active, sk, jab, jcd, kac, kad, kbc, kbd);    \
     if (sk) {                                                                                     \
       __syncthreads();                                                                            \
-      for (int i = threadIdx.x; i < (NA + NB) * n_; i += QX_BLOCK) {                              \
+      for (int i = threadIdx.x; i < (NA + NB) * n_; i += bs_) {                                   \
         const double v = sk[i];                                                                   \
         if (v != 0.0) {                                                                           \
           const int r = i / n_, col = i - r * n_;                                                 \
@@ -654,12 +663,14 @@ __device__ __forceinline__ void qx_epi_djk(
     b.e.krows = a->e.krows && ((A1 - A0) * (NC + ND) + NB * NC + NB * ND) >= QX_JK_ROWS_MIN &&    \
                 rows_bytes <= QX_JK_LDS_MAX;                                                      \
     const size_t sh = b.e.krows ? rows_bytes : 0;                                                 \
+    const int bs = qx_launch_block(L);                                                            \
+    if (bs % 64 || bs > QX_BLOCK_MAX) return (int)hipErrorInvalidValue;                           \
+    const dim3 grid((unsigned)((a->w.nquart + bs - 1) / bs));                                     \
     if (L->fp32) {                                                                                \
       if (!(WITH32)) return -1;                                                                   \
-      if constexpr (WITH32)                                                                       \
-        hipLaunchKernelGGL(qx_kj_##NAME<float>, QX_GRID(a->w.nquart), dim3(QX_BLOCK), sh, L->stream, b); \
+      if constexpr (WITH32) hipLaunchKernelGGL(qx_kj_##NAME<float>, grid, dim3(bs), sh, L->stream, b); \
     } else {                                                                                      \
-      hipLaunchKernelGGL(qx_kj_##NAME<double>, QX_GRID(a->w.nquart), dim3(QX_BLOCK), sh, L->stream, b); \
+      hipLaunchKernelGGL(qx_kj_##NAME<double>, grid, dim3(bs), sh, L->stream, b);                \
     }                                                                                             \
     return (int)hipGetLastError();                                                                \
   }
@@ -754,13 +765,14 @@ template <class Part>
 __device__ __forceinline__ void qx_grad_body(const QxArgs& A_, double* sg, Part part) {
   const int ng = 3 * A_.e.natom;
   const bool lds = ng <= QX_GRAD_LDS;
-  const long long t = (long long)blockIdx.x * QX_BLOCK + threadIdx.x;
+  const int bs = (int)blockDim.x;
+  const long long t = (long long)blockIdx.x * bs + threadIdx.x;
   int u1 = 0, u2 = 0, P1 = -1, P2 = -1;
   bool active = t < A_.w.nquart;
   if (active) active = qx_locate(A_, t, u1, u2, P1, P2);
   if (qx_block_idle(A_, active)) return;
   if (lds) {
-    for (int i = threadIdx.x; i < ng; i += QX_BLOCK) sg[i] = 0.0;
+    for (int i = threadIdx.x; i < ng; i += bs) sg[i] = 0.0;
     __syncthreads();
   }
   if (active) {
@@ -782,7 +794,7 @@ __device__ __forceinline__ void qx_grad_body(const QxArgs& A_, double* sg, Part 
   }
   if (lds) {
     __syncthreads();
-    for (int i = threadIdx.x; i < ng; i += QX_BLOCK)
+    for (int i = threadIdx.x; i < ng; i += bs)
       if (sg[i] != 0.0) qx_atomic_add(&A_.e.grad[i], sg[i]);
   }
 }
@@ -792,12 +804,14 @@ __device__ __forceinline__ void qx_grad_body(const QxArgs& A_, double* sg, Part 
   extern "C" int qx_launchg_##NAME(const QxArgs* a, const QxLaunch* L) {                          \
     if (a->w.nquart <= 0) return 0;                                                               \
     if (a->e.epi != QX_EPI_GRAD) return -1;                                                       \
-    const dim3 grid(QX_GRID(a->w.nquart).x, NPARTS);                                              \
+    const int bs = qx_launch_block(L);                                                            \
+    if (bs % 64 || bs > QX_BLOCK_MAX) return (int)hipErrorInvalidValue;                           \
+    const dim3 grid((unsigned)((a->w.nquart + bs - 1) / bs), NPARTS);                             \
     if (L->fp32) {                                                                                \
       if (!(WITH32)) return -1;                                                                   \
-      if constexpr (WITH32) hipLaunchKernelGGL(qx_kg_##NAME<float>, grid, dim3(QX_BLOCK), 0, L->stream, *a); \
+      if constexpr (WITH32) hipLaunchKernelGGL(qx_kg_##NAME<float>, grid, dim3(bs), 0, L->stream, *a); \
     } else {                                                                                      \
-      hipLaunchKernelGGL(qx_kg_##NAME<double>, grid, dim3(QX_BLOCK), 0, L->stream, *a);           \
+      hipLaunchKernelGGL(qx_kg_##NAME<double>, grid, dim3(bs), 0, L->stream, *a);                 \
     }                                                                                             \
     return (int)hipGetLastError();                                                                \
   }

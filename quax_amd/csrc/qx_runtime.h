@@ -115,6 +115,10 @@ struct qx_ctx {
   // stream order sequences its passes)
   bool capturing = false;
 
+  // threads per block of the streaming kernels, per pass (0 J/K, 1 adjoint
+  // gradient) and class (the tuned quartet tiles)
+  int tile[2][QX_NUM_CLASSES_MAX] = {};
+  int* d_pair_frag = nullptr;            // fragment of each sorted pair (fragmented contexts)
   // mixed precision: per class and pass (0 J/K, 1 gradient) whether quartets
   // below the threshold run in FP32 (classes whose FP32 kernels are faster)
   unsigned char fp32_ok[2][QX_NUM_CLASSES_MAX] = {};

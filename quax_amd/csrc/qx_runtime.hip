@@ -19,7 +19,7 @@
 
 #include "quax_amd.h"
 #include "qx_runtime.h"
-#include "_gen/qx_classes.inc"
+#include "qx_classes.inc"  // generated; in the build's generated-source directory
 
 static_assert(QX_GEN_LMAX == QX_LMAX_SUPPORTED, "generated kernels and runtime disagree on lmax");
 static_assert(QX_NUM_CLASSES == QX_NUM_CLASSES_MAX, "profile slot layout");
@@ -498,7 +498,10 @@ void schwarz_and_lists(qx_ctx* c) {
     }
   }
   c->d_prefix = dupload(c, allpre);
-  if (c->fragmented) c->d_kstart = dupload(c, allkst);
+  if (c->fragmented) {
+    c->d_kstart = dupload(c, allkst);
+    c->d_pair_frag = dupload(c, c->h_pair_frag);
+  }
   c->h_prefix = allpre;
   c->h_kstart = allkst;
   c->h_kstart_off = kst_off;
@@ -689,7 +692,10 @@ static int init_impl(int natom, const double* xyz_bohr, const double* nuclear_ch
     try {
       c = new qx_ctx();
       for (int k = 0; k < QX_NUM_CLASSES; ++k)
-        for (int pass = 0; pass < 2; ++pass) c->fp32_ok[pass][k] = qx_fp32_default[k][pass];
+        for (int pass = 0; pass < 2; ++pass) {
+          c->fp32_ok[pass][k] = qx_fp32_default[k][pass];
+          c->tile[pass][k] = qx_tile_default[k][pass];
+        }
       init_body(c, natom, xyz_bohr, nuclear_charges, nshell, shell_l, shell_atom, shell_nprim, exps, coefs, pure,
                 ints_tol, device, atom_fragment);
     } catch (...) {
@@ -901,8 +907,9 @@ static void add_deriv_list(qx_ctx* c, const QxWorkHost& wh, const QxEpiDev& e, i
     const int slot = QX_SLOT_DERIV(wh.cls, k, fp32 ? 1 : 0);
     auto fn = grad ? ent.grad[k] : ent.deri[k];
     const double fl = grad ? ent.group_flops[k] : ent.part_flops[k];
-    out.push_back({(double)nq * fl, [c, a, fn, fp32, slot, nq](hipStream_t st) {
-                     QxLaunch L{fp32, st};
+    const int block = grad ? c->tile[1][wh.cls] : 0;
+    out.push_back({(double)nq * fl, [c, a, fn, fp32, slot, nq, block](hipStream_t st) {
+                     QxLaunch L{fp32, st, block};
                      return timed_launch(c, slot, nq, st, [&] { return fn(&a, &L); });
                    }});
   }
@@ -919,8 +926,9 @@ static void add_jk_list(qx_ctx* c, const QxWorkHost& wh, const QxEpiDev& e, int 
     a.w.sel = sel;
     const int slot = QX_SLOT_JK(wh.cls, k, fp32 ? 1 : 0);
     auto fn = ent.jk[k];
-    out.push_back({(double)nq * ent.jk_flops[k], [c, a, fn, fp32, slot, nq](hipStream_t st) {
-                     QxLaunch L{fp32, st};
+    const int block = c->tile[0][wh.cls];
+    out.push_back({(double)nq * ent.jk_flops[k], [c, a, fn, fp32, slot, nq, block](hipStream_t st) {
+                     QxLaunch L{fp32, st, block};
                      return timed_launch(c, slot, nq, st, [&] { return fn(&a, &L); });
                    }});
   }
@@ -1105,7 +1113,13 @@ int qx_compute_2e_deriv(qx_ctx* c, const int* dv, int len, double* out, void* st
 }
 
 int qx_jk_build(qx_ctx* c, const double* D, double* J, double* K, int precision, double thr, void* stream) {
+  return qx_jk_build_masked(c, D, J, K, precision, thr, nullptr, stream);
+}
+
+int qx_jk_build_masked(qx_ctx* c, const double* D, double* J, double* K, int precision, double thr,
+                       const int* frag_skip, void* stream) {
   if (!c || !D || !J || !K) return QX_EINVAL;
+  if (frag_skip && !c->fragmented) return QX_EINVAL;
   if (precision != QX_PREC_FP64 && precision != QX_PREC_MIXED) return QX_EINVAL;
   if (precision == QX_PREC_MIXED && !(thr >= 0.0 && std::isfinite(thr))) return QX_EINVAL;
   return guarded(c, [&]() -> int {
@@ -1124,6 +1138,10 @@ int qx_jk_build(qx_ctx* c, const double* D, double* J, double* K, int precision,
     QxEpiDev e{};
     e.epi = QX_EPI_JK; e.nbf = (int)nc; e.natom = c->natom; e.coord = -1; e.D = Dcart; e.J = Jp; e.K = Kp;
     e.krows = 1;  // each class kernel decides whether its K' rows fit in LDS
+    if (frag_skip) {
+      e.frag_skip = frag_skip;
+      e.pair_frag = c->d_pair_frag;
+    }
     std::vector<Launch> v;
     if (precision == QX_PREC_MIXED) {
       add_mixed_launches(c, thr, 0, e, v);
@@ -1489,6 +1507,21 @@ int qx_mixed_pass_quartets(qx_ctx* c, double thr, int pass, long long* n64, long
     if (n32) *n32 = b;
     return QX_OK;
   });
+}
+
+int qx_set_class_tiles(qx_ctx* c, int pass, const int* block) {
+  if (!c || pass < 0 || pass > 1) return QX_EINVAL;
+  if (block)
+    for (int k = 0; k < QX_NUM_CLASSES; ++k)
+      if (block[k] < 64 || block[k] > QX_BLOCK_MAX || block[k] % 64) return QX_EINVAL;
+  for (int k = 0; k < QX_NUM_CLASSES; ++k) c->tile[pass][k] = block ? block[k] : qx_tile_default[k][pass];
+  return QX_OK;
+}
+
+int qx_get_class_tiles(const qx_ctx* c, int pass, int* block) {
+  if (!c || !block || pass < 0 || pass > 1) return QX_EINVAL;
+  for (int k = 0; k < QX_NUM_CLASSES; ++k) block[k] = c->tile[pass][k];
+  return QX_OK;
 }
 
 int qx_set_fp32_classes(qx_ctx* c, int pass, const int* flags) {

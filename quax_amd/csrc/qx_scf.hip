@@ -444,9 +444,10 @@ __global__ __launch_bounds__(SB) void k_mcweeny(const double* X, const double* X
   Xout[o] = 3.0 * X2[o] - 2.0 * X3[o];
 }
 
-// end of an iteration: energy change, counters, convergence (grid 1, block
-// >= nmol)
-__global__ void k_finish(double* sc, int nmol, double conv, int maxit) {
+// end of an iteration: energy change, counters, convergence; skip[b] (the
+// J/K pass's per-fragment mask) follows the done flag (grid ceil(nmol / 64),
+// block 64)
+__global__ void k_finish(double* sc, int* skip, int nmol, double conv, int maxit) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nmol || mol_done(sc, b)) return;
   double* s = sc + b * QS_N;
@@ -454,7 +455,10 @@ __global__ void k_finish(double* sc, int nmol, double conv, int maxit) {
   s[QS_E] = s[QS_ENEW];
   s[QS_IT] += 1.0;
   const bool conv_ok = !(fabs(s[QS_DE]) > conv || s[QS_DRMS] > conv);
-  if (conv_ok || s[QS_IT] >= maxit) s[QS_DONE] = 1.0;
+  if (conv_ok || s[QS_IT] >= maxit) {
+    s[QS_DONE] = 1.0;
+    skip[b] = 1;
+  }
 }
 
 }  // namespace
@@ -467,6 +471,7 @@ struct qx_scf {
   std::vector<int> h_nb;
   int* nb = nullptr;
   int* off = nullptr;
+  int* skip = nullptr;          // per molecule (= fragment id): 1 once done; masks the J/K pass of batches
   // B x nm x nm
   double *H = nullptr, *S = nullptr, *A = nullptr;
   double *D = nullptr, *Dold = nullptr, *Dnew = nullptr, *F = nullptr, *Fe = nullptr, *J = nullptr, *K = nullptr;
@@ -545,7 +550,9 @@ void gemm(qx_scf* s, bool ta, bool tb, const double* A, const double* B, double*
 void jk(qx_scf* s, const double* Dsrc, hipStream_t st) {
   hipLaunchKernelGGL(k_scatter, egrid(s), dim3(SB), 0, st, Dsrc, s->Dfull, s->ntot, s->nm, s->nb, s->off);
   check(hipGetLastError(), "scf scatter");
-  const int rc = qx_jk_build(s->ctx, s->Dfull, s->Jfull, s->Kfull, s->precision, s->thr, st);
+  const int rc = s->nmol > 1
+                     ? qx_jk_build_masked(s->ctx, s->Dfull, s->Jfull, s->Kfull, s->precision, s->thr, s->skip, st)
+                     : qx_jk_build(s->ctx, s->Dfull, s->Jfull, s->Kfull, s->precision, s->thr, st);
   if (rc != QX_OK) throw ScfError{rc, std::string("scf J/K: ") + qx_last_error(s->ctx)};
   hipLaunchKernelGGL(k_gather, egrid(s), dim3(SB), 0, st, s->Jfull, s->ntot, s->J, s->nm, s->nb, s->off, s->sc, 1);
   hipLaunchKernelGGL(k_gather, egrid(s), dim3(SB), 0, st, s->Kfull, s->ntot, s->K, s->nm, s->nb, s->off, s->sc, 1);
@@ -613,8 +620,8 @@ void iteration(qx_scf* s, hipStream_t st) {
   density(s, Fuse, st);
   hipLaunchKernelGGL(k_copy, egrid(s), dim3(SB), 0, st, s->D, s->Dold, s->nm, s->sc, 1);
   hipLaunchKernelGGL(k_copy, egrid(s), dim3(SB), 0, st, s->Dnew, s->D, s->nm, s->sc, 1);
-  hipLaunchKernelGGL(k_finish, dim3((unsigned)((s->nmol + 63) / 64)), dim3(64), 0, st, s->sc, s->nmol, s->conv,
-                     s->maxit);
+  hipLaunchKernelGGL(k_finish, dim3((unsigned)((s->nmol + 63) / 64)), dim3(64), 0, st, s->sc, s->skip, s->nmol,
+                     s->conv, s->maxit);
   check(hipGetLastError(), "scf finish");
 }
 
@@ -638,6 +645,16 @@ int qx_scf_create(qx_ctx* ctx, int nmol, int nm, const int* nbf, const int* offs
     if (nbf[b] < 1 || nbf[b] > nm || offset[b] < 0 || offset[b] + nbf[b] > ntot) return QX_EINVAL;
     if (nocc[b] < 0 || nocc[b] > nbf[b]) return QX_EINVAL;
   }
+  if (nmol > 1) {
+    // a batch: molecule b is fragment b of ctx and owns AOs [offset[b], offset[b] + nbf[b])
+    if (!ctx->fragmented) return QX_EINVAL;
+    for (int sh = 0; sh < ctx->nshell; ++sh) {
+      const int f = ctx->frag[ctx->atom[sh]], l = ctx->l[sh];
+      const int size = ctx->pure ? 2 * l + 1 : (l + 1) * (l + 2) / 2;
+      if (f < 0 || f >= nmol || ctx->bfs[sh] < offset[f] || ctx->bfs[sh] + size > offset[f] + nbf[f])
+        return QX_EINVAL;
+    }
+  }
   qx_scf* s = new (std::nothrow) qx_scf();
   if (!s) return QX_ENOMEM;
   const int rc = scf_guarded(nullptr, [&]() -> int {
@@ -648,6 +665,8 @@ int qx_scf_create(qx_ctx* ctx, int nmol, int nm, const int* nbf, const int* offs
     const size_t nb2 = (size_t)nmol * nm * nm;
     s->nb = salloc<int>(s, nmol);
     s->off = salloc<int>(s, nmol);
+    s->skip = salloc<int>(s, nmol);
+    QS_HIP(hipMemset(s->skip, 0, nmol * sizeof(int)));
     QS_HIP(hipMemcpy(s->nb, nbf, nmol * sizeof(int), hipMemcpyHostToDevice));
     QS_HIP(hipMemcpy(s->off, offset, nmol * sizeof(int), hipMemcpyHostToDevice));
     for (double** p : {&s->H, &s->S, &s->A, &s->D, &s->Dold, &s->Dnew, &s->F, &s->Fe, &s->J, &s->K, &s->T1, &s->T2,
@@ -724,6 +743,8 @@ int qx_scf_init(qx_scf* s, const double* D0, void* stream) {
       sc[b * QS_N + QS_DRMS] = 1.0;
     }
     QS_HIP(hipMemcpy(s->sc, sc.data(), sc.size() * sizeof(double), hipMemcpyHostToDevice));
+    std::vector<int> sk(s->nmol, s->maxit == 0 ? 1 : 0);
+    QS_HIP(hipMemcpy(s->skip, sk.data(), sk.size() * sizeof(int), hipMemcpyHostToDevice));
     s->initialised = true;
     return QX_OK;
   });

@@ -1,7 +1,19 @@
-"""Derivative-part autotuner: picks, per angular-momentum class, the node
-budget at which gen_eri.py splits the adjoint-gradient kernel (--key budget)
-or the waves per SIMD its kernels are compiled for (--key waves; a library
-built with --waves W, W = 0 for the compiler's choice).
+"""Autotuner of the per-class kernel choices, from GPU timings.
+
+--tiles (in process, one library): the quartet tile of every class -- the
+threads per workgroup of its streaming J/K and adjoint-gradient kernels,
+i.e. how many primitive quartets a workgroup evaluates and how many share its
+LDS K' rows -- out of 64/128/192/256, timed per class on one stream, and
+the classes whose FP32 kernels beat their FP64 ones by 8% or more (the
+mixed-precision policy).  Writes the "tiles" and "fp32" keys that gen_eri.py
+compiles into the default tables (qx_set_class_tiles / qx_set_fp32_classes
+override them at run time):
+    python -m quax_amd.csrc.tune --tiles --merge quax_amd/csrc/deriv_budget.json --out tiles.json
+
+Otherwise (several libraries): the node budget at which gen_eri.py splits
+the adjoint-gradient kernel (--key budget) or the waves per SIMD its kernels
+are compiled for (--key waves; a library built with --waves W, W = 0 for the
+compiler's choice).
 
 A smaller budget splits a class into more parts: each part recomputes the
 recurrence nodes it shares with the others (more FP64 work) but keeps its
@@ -48,7 +60,150 @@ def measure(lib_path: str, workload: str, steps: int, timeout: int) -> dict:
     return json.loads(r.stdout.strip().splitlines()[-1])
 
 
+TILE_CHOICES = (64, 128, 192, 256)
+FP32_GAIN = 0.92   # FP32 admitted when its kernels take at most this fraction of the FP64 time
+
+
+def _class_names(L):
+    import ctypes
+    names = []
+    for c in range(L.qx_num_classes()):
+        l4 = (ctypes.c_int * 4)()
+        npart = ctypes.c_int()
+        L.qx_class_info(c, l4, ctypes.byref(npart))
+        names.append("".join(str(x) for x in l4))
+    return names
+
+
+def _pass_times(eng, D, steps, precision=0, thr=0.0):
+    """Kernel ms per step per (class name, pass) from HIP-event profiling on
+    one stream; pass 0 = J/K kernels, 1 = adjoint-gradient kernels."""
+    import ctypes
+    import torch
+    L = eng.L
+    names = _class_names(L)
+    J, K = torch.empty_like(D), torch.empty_like(D)
+
+    def step():
+        eng.jk(D, precision=precision, fp32_threshold=thr, out=(J, K))
+        eng.eri_grad(D, precision=precision, fp32_threshold=thr)
+
+    step()
+    torch.cuda.synchronize()
+    eng.profile(True)
+    for _ in range(steps):
+        step()
+    ms, _, _ = eng.profile_read()
+    eng.profile(False)
+    out = {}
+    for slot, t in enumerate(ms):
+        if t <= 0:
+            continue
+        kind, cls, part, f32 = (ctypes.c_int() for _ in range(4))
+        L.qx_profile_slot(slot, ctypes.byref(kind), ctypes.byref(cls), ctypes.byref(part), ctypes.byref(f32))
+        if kind.value not in (1, 2):
+            continue
+        key = (names[cls.value], 0 if kind.value == 2 else 1)
+        out[key] = out.get(key, 0.0) + float(t) / steps
+    return out
+
+
+def tune_tiles(workload, steps, seed=0):
+    import numpy as np
+    import torch
+    from quax_amd import workloads
+    from quax_amd.basis import BasisSet
+    from quax_amd.engine import EriEngine
+    mol = getattr(workloads, workload)()
+    bs = BasisSet.build(mol, "even-tempered", pure=True)
+    eng = EriEngine(mol, bs, ints_tol=1e-12)
+    n = eng.nbf
+    X = np.random.default_rng(seed).standard_normal((n, n)) * 0.3
+    D = torch.as_tensor(X @ X.T / n, device=eng.device)
+    nc = eng.L.qx_num_classes()
+    names = _class_names(eng.L)
+    eng.set_streams(1)
+    per = {}
+    for b in TILE_CHOICES:
+        for p in (0, 1):
+            eng.set_class_tiles(p, [b] * nc)
+        per[b] = _pass_times(eng, D, steps)
+        print(f"[tune] tile {b}: J/K {sum(v for (c, p), v in per[b].items() if p == 0):.2f} ms, adjoint "
+              f"{sum(v for (c, p), v in per[b].items() if p == 1):.2f} ms per step (one stream)", flush=True)
+    tiles = {"jk": {}, "grad": {}}
+    table = {}
+    for (cls, p) in sorted(set().union(*[set(v) for v in per.values()])):
+        times = {b: per[b][(cls, p)] for b in per if (cls, p) in per[b]}
+        best = min(times, key=times.get)
+        # keep the default unless another tile is at least 2% faster
+        if times.get(128, 1e300) <= 1.02 * times[best]:
+            best = 128
+        tiles["jk" if p == 0 else "grad"][cls] = best
+        table[f"{cls}/{'jk' if p == 0 else 'grad'}"] = times
+        print(f"[tune] {cls} {'J/K ' if p == 0 else 'grad'}: " + "  ".join(f"{b}:{times[b]:.3f}" for b in sorted(times))
+              + f"  -> {best}", flush=True)
+    for p, key in ((0, "jk"), (1, "grad")):
+        eng.set_class_tiles(p, [tiles[key].get(names[c], 128) for c in range(nc)])
+    # FP32 policy: every quartet of every class in FP32 against FP64, at the chosen tiles
+    t64 = _pass_times(eng, D, steps)
+    for p in (0, 1):
+        eng.set_fp32_classes(p, np.ones(nc, dtype=bool))
+    t32 = _pass_times(eng, D, steps, precision=1, thr=1e300)
+    fp32 = {"jk": [], "grad": []}
+    for (cls, p), t in sorted(t64.items()):
+        if (cls, p) in t32 and t32[(cls, p)] <= FP32_GAIN * t:
+            fp32["jk" if p == 0 else "grad"].append(cls)
+    print(f"[tune] FP32 admitted: J/K {fp32['jk']}, gradient {fp32['grad']}", flush=True)
+    # the whole step on four streams: tuned tiles against the 128 default
+    eng.set_streams(4)
+    for p in (0, 1):
+        eng.set_fp32_classes(p, None)
+    res = {}
+    for label, tl in (("default", None), ("tuned", tiles)):
+        for p, key in ((0, "jk"), (1, "grad")):
+            eng.set_class_tiles(p, [128] * nc if tl is None else [tl[key].get(names[c], 128) for c in range(nc)])
+        J, K = torch.empty_like(D), torch.empty_like(D)
+        for _ in range(2):
+            eng.jk(D, out=(J, K))
+            eng.eri_grad(D)
+        torch.cuda.synchronize()
+        import time
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            eng.jk(D, out=(J, K))
+            eng.eri_grad(D)
+        torch.cuda.synchronize()
+        res[label] = (time.perf_counter() - t0) / steps * 1e3
+        print(f"[tune] four streams, {label} tiles: {res[label]:.2f} ms per step", flush=True)
+    eng.close()
+    return {"tiles": {k: {c: b for c, b in v.items() if b != 128} for k, v in tiles.items()}, "fp32": fp32,
+            "ms_per_step_tiles": {"per_class_one_stream": table, "four_streams": res}}
+
+
 def main():
+    if "--tiles" in sys.argv:
+        ap = argparse.ArgumentParser()
+        ap.add_argument("--tiles", action="store_true")
+        ap.add_argument("--workload", default="cluster10")
+        ap.add_argument("--steps", type=int, default=3)
+        ap.add_argument("--out", default="tiles.json")
+        ap.add_argument("--merge", default=None, help="existing tuning JSON whose other keys are kept")
+        a = ap.parse_args()
+        res = tune_tiles(a.workload, a.steps)
+        out = {}
+        if a.merge and os.path.exists(a.merge):
+            with open(a.merge) as fh:
+                out = json.load(fh)
+        fp_note = out.get("fp32", {}).get("note")
+        out.update(res)
+        out["fp32"]["note"] = (f"classes whose FP32 kernels took <= {FP32_GAIN:.2f} of the FP64 time "
+                               f"({a.workload}, one stream, whole lists, tuned tiles)")
+        del fp_note
+        out["tiles_workload"] = a.workload
+        with open(a.out, "w") as fh:
+            json.dump(out, fh, indent=1, sort_keys=True)
+        print(f"[tune] wrote {a.out}")
+        return
     ap = argparse.ArgumentParser()
     ap.add_argument("libs", nargs="+", help="one library per budget, in directories named by the budget")
     ap.add_argument("--workload", default="cluster10")

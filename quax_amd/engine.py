@@ -139,6 +139,25 @@ class EriEngine:
         check(self.L.qx_get_fp32_classes(self.ctx, int(pass_), f.ctypes.data_as(ctypes.c_void_p)), self.ctx)
         return f.astype(bool)
 
+    def class_tiles(self, pass_: int):
+        """Per kernel class, threads per block of the streaming kernels of a
+        pass (0 J/K, 1 adjoint gradient)."""
+        f = np.zeros(self.L.qx_num_classes(), dtype=np.int32)
+        check(self.L.qx_get_class_tiles(self.ctx, int(pass_), f.ctypes.data_as(ctypes.c_void_p)), self.ctx)
+        return f
+
+    def set_class_tiles(self, pass_: int, block=None):
+        """Sets the per-class threads per block (multiples of 64 up to 256;
+        None: the built-in tuned table)."""
+        if block is None:
+            check(self.L.qx_set_class_tiles(self.ctx, int(pass_), None), self.ctx)
+            return
+        f = np.ascontiguousarray(np.asarray(block, dtype=np.int32))
+        if f.shape != (self.L.qx_num_classes(),):
+            raise ValueError("one block size per kernel class")
+        check(self.L.qx_set_class_tiles(self.ctx, int(pass_), f.ctypes.data_as(ctypes.c_void_p)), self.ctx,
+              "set_class_tiles")
+
     def set_fp32_classes(self, pass_: int, flags=None):
         """Sets the per-class FP32 policy of a mixed pass (None: built-in default)."""
         if flags is None:
@@ -223,12 +242,20 @@ class EriEngine:
         return lists, np.stack([sa, sb], axis=1)
 
     # ------------------------------------------------------------ direct
-    def jk(self, D, precision: int = QX_PREC_FP64, fp32_threshold: float = 0.0, out=None):
+    def jk(self, D, precision: int = QX_PREC_FP64, fp32_threshold: float = 0.0, out=None, frag_skip=None):
+        """J and K of D.  frag_skip (batch contexts): per fragment id, nonzero
+        = leave that fragment's quartets out (its blocks come out zero)."""
         D = self._dev(D)
         n = self.nbf
         J, K = (out if out is not None else (self._empty(n, n), self._empty(n, n)))
-        check(self.L.qx_jk_build(self.ctx, ptr(D), ptr(J), ptr(K), int(precision), float(fp32_threshold),
-                                 self._stream()), self.ctx, "jk_build")
+        if frag_skip is None:
+            check(self.L.qx_jk_build(self.ctx, ptr(D), ptr(J), ptr(K), int(precision), float(fp32_threshold),
+                                     self._stream()), self.ctx, "jk_build")
+        else:
+            sk = self.torch.as_tensor(np.asarray(frag_skip, dtype=np.int32), device=self.device)
+            check(self.L.qx_jk_build_masked(self.ctx, ptr(D), ptr(J), ptr(K), int(precision),
+                                            float(fp32_threshold), ptr(sk), self._stream()), self.ctx,
+                  "jk_build_masked")
         return J, K
 
     def eri_grad(self, D, precision: int = QX_PREC_FP64, fp32_threshold: float = 0.0):

@@ -92,6 +92,37 @@ def compute_2e_deriv(type_: str, beta: float, deriv_vec) -> np.ndarray:
     return _get().compute_2e_deriv(dv).reshape(-1).cpu().numpy()
 
 
+def compute_2e_deriv_disk(type_: str, beta: float, max_deriv_order: int) -> None:
+    """libint_interface.cc:1811-2115: writes every derivative tensor of
+    orders 1..max_deriv_order to <type>_derivs.h5 in the working directory,
+    one dataset "<type>_<n>_<n>_<n>_<n>_deriv<order>" of shape (n, n, n, n,
+    how_many_derivs) per order (derivative index last, zero fill).  The
+    tensors come from eri_deriv_core on the GPU, transposed there and written
+    one first-index row at a time (libquax_h5.so).  Like the reference, a
+    request above 50 GB of tensors is refused (ValueError here, an assert
+    there)."""
+    from .. import _h5
+    _check_type(type_)
+    if max_deriv_order < 1 or max_deriv_order > LIBINT2_MAX_DERIV_ORDER:
+        raise NotImplementedError(f"compute_2e_deriv_disk: orders 1 to {LIBINT2_MAX_DERIV_ORDER} are built")
+    eng = _get()
+    n = eng.nbf
+    total = sum(how_many_derivs(_natom, k) for k in range(1, max_deriv_order + 1))
+    if n ** 4 * total * 8 * 1e-9 >= 50:
+        raise ValueError("Total disk space required for ERI's exceeds 50 GB.")
+    path = f"{type_}_derivs.h5"
+    mode = _h5.TRUNCATE
+    for order in range(1, max_deriv_order + 1):
+        nd = how_many_derivs(_natom, order)
+        T = eng.eri_deriv_core(order).reshape(nd, n, n, n, n)
+        name = f"{type_}_{n}_{n}_{n}_{n}_deriv{order}"
+        for i in range(n):
+            row = T[:, i].permute(1, 2, 3, 0).contiguous().cpu().numpy()
+            _h5.write_block(path, name, (n, n, n, n, nd), i, row[None], mode=mode)
+            mode = _h5.APPEND
+        del T
+
+
 def generate_multi_index_lookup(nparams: int, deriv_order: int):
     """utils.h:142-172: combinations with replacement, lexicographic."""
     import ctypes

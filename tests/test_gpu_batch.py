@@ -91,3 +91,103 @@ def test_batched_rhf_matches_single_molecule_runs(gpu):
             gk = rhf_gradient(e1, auxk)
         assert abs(E[k] - Ek) < 1e-9
         assert np.abs(G[k] - gk).max() < 1e-7
+
+
+def test_masked_jk_leaves_out_fragments(gpu):
+    """qx_jk_build_masked: the blocks of masked fragments come out zero, the
+    others equal the unmasked pass."""
+    from quax_amd._lib import QuaxNativeError
+    from quax_amd.engine import EriEngine
+    mols = _smallest(3)
+    bases = [BasisSet.build(m, "even-tempered", pure=True) for m in mols]
+    comb, cb, frag = _combined(mols, bases)
+    nb = [b.nbf() for b in bases]
+    off = np.concatenate([[0], np.cumsum(nb)]).astype(int)
+    rng = np.random.default_rng(8)
+    D = np.zeros((off[-1], off[-1]))
+    for k, n in enumerate(nb):
+        X = rng.standard_normal((n, n)) * 0.3
+        D[off[k]:off[k + 1], off[k]:off[k + 1]] = X @ X.T / n
+    with EriEngine(comb, cb, ints_tol=1e-12, fragments=frag) as eng:
+        J, K = (x.cpu().numpy() for x in eng.jk(D))
+        for mask in ([0, 1, 0], [1, 0, 1], [1, 1, 1], [0, 0, 0]):
+            Jm, Km = (x.cpu().numpy() for x in eng.jk(D, frag_skip=mask))
+            for k in range(3):
+                s = slice(off[k], off[k + 1])
+                if mask[k]:
+                    assert np.abs(Jm[s, s]).max() == 0.0 and np.abs(Km[s, s]).max() == 0.0
+                else:
+                    assert np.abs(Jm[s, s] - J[s, s]).max() < 1e-12
+                    assert np.abs(Km[s, s] - K[s, s]).max() < 1e-12
+    # a context without fragments takes no mask
+    m = mols[0]
+    with EriEngine(m, bases[0], ints_tol=1e-12) as e1:
+        with pytest.raises(QuaxNativeError):
+            e1.jk(D[:nb[0], :nb[0]], frag_skip=[0])
+
+
+def test_class_tiles_do_not_change_results(gpu):
+    """Every quartet tile (threads per workgroup) of the streaming kernels
+    gives the same J, K and gradient; invalid tiles are refused."""
+    from quax_amd._lib import QuaxNativeError
+    from quax_amd.engine import EriEngine
+    m = workloads.cluster10()
+    mol = Molecule([m.symbol(i) for i in range(3)], m.geometry()[:3])
+    bs = BasisSet.build(mol, "even-tempered", pure=True)
+    with EriEngine(mol, bs, ints_tol=1e-12) as eng:
+        n = eng.nbf
+        X = np.random.default_rng(3).standard_normal((n, n)) * 0.3
+        D = X @ X.T / n
+        default = [eng.class_tiles(p) for p in (0, 1)]
+        assert all(((t >= 64) & (t <= 256) & (t % 64 == 0)).all() for t in default)
+        J, K = (x.cpu().numpy() for x in eng.jk(D))
+        g = eng.eri_grad(D).cpu().numpy()
+        nc = len(default[0])
+        rng = np.random.default_rng(5)
+        for tiles in ([64] * nc, [192] * nc, [256] * nc, list(rng.choice([64, 128, 192, 256], nc))):
+            for p in (0, 1):
+                eng.set_class_tiles(p, tiles)
+            Jt, Kt = (x.cpu().numpy() for x in eng.jk(D))
+            gt = eng.eri_grad(D).cpu().numpy()
+            assert np.abs(Jt - J).max() < 1e-12 and np.abs(Kt - K).max() < 1e-12
+            assert np.abs(gt - g).max() < 1e-12
+        for p in (0, 1):
+            eng.set_class_tiles(p, None)
+            assert (eng.class_tiles(p) == default[p]).all()
+        for bad in ([32] * nc, [320] * nc, [100] * nc):
+            with pytest.raises(QuaxNativeError):
+                eng.set_class_tiles(0, bad)
+
+
+def test_batched_graph_scf_seeded_molecules(gpu):
+    """Seeded batch molecules (3-12 atoms, C/H/O) through the graph SCF
+    (converged molecules masked out of J/K) against each molecule alone
+    through the eager driver: energies 1e-9 Eh, gradients 1e-7.  Only
+    molecules whose eager SCF converges take part: from the zero guess DIIS
+    wanders without converging on some synthetic geometries (one of the first
+    five is such a case on the GPU and in numpy alike, where tiny rounding
+    differences decide the path), and the graph SCF follows the same
+    iteration."""
+    from quax_amd.engine import EriEngine
+    from quax_amd.methods.batch import batched_rhf
+    from quax_amd.methods.hartree_fock import restricted_hartree_fock, rhf_gradient
+    opts = {"diis": True, "spectral_shift": False, "conv": 1e-10, "maxit": 200}
+    ref = []
+    for m in _well_behaved() + _smallest(8):
+        b = BasisSet.build(m, "even-tempered", pure=True)
+        with EriEngine(m, b, ints_tol=1e-12) as e1:
+            Ek, auxk = restricted_hartree_fock(e1, m.nelectron(), opts, return_aux_data=True)
+            if auxk["drms"] > 1e-10 or abs(auxk["dE"]) > 1e-10:
+                print(f"  {m.natom()} atoms: eager SCF did not converge, left out", flush=True)
+                continue
+            ref.append((m, Ek, rhf_gradient(e1, auxk)))
+    assert len(ref) >= 6
+    mols = [r[0] for r in ref]
+    E, G, aux = batched_rhf(mols, "even-tempered", pure=True, ints_tol=1e-12, conv=1e-10, graph=True)
+    print(f"  graph batch: iterations {aux['iterations']}, converged {aux['converged']}", flush=True)
+    assert all(aux["converged"])
+    for k, (m, Ek, gk) in enumerate(ref):
+        print(f"  molecule {k} ({m.natom()} atoms): dE {E[k] - Ek:.1e}, max dg {np.abs(G[k] - gk).max():.1e}",
+              flush=True)
+        assert abs(E[k] - Ek) < 1e-9
+        assert np.abs(G[k] - gk).max() < 1e-7

@@ -246,3 +246,33 @@ def test_fock_eigensolver_occupied_subset():
     assert torch.allclose(vs @ vs.T, v[:, :12] @ v[:, :12].T, atol=1e-12)
     wf, vf = _eigh(F, "host", nocc=60)      # nocc = n falls back to the full solve
     assert wf.shape == (60,)
+
+
+def test_tuned_tables_reach_the_generated_class_table(tmp_path):
+    """The autotuner's JSON (quartet tiles, FP32 policy) becomes the default
+    tables the runtime compiles in: qx_tile_default[cls] = {J/K, gradient}
+    threads per block, qx_fp32_default[cls] = {J/K, gradient} flags."""
+    import json
+    import subprocess
+    import sys
+    tuning = {"tiles": {"jk": {"1110": 64}, "grad": {"1111": 256, "1000": 192}},
+              "fp32": {"jk": ["1111"], "grad": ["1110", "1111"]}}
+    f = tmp_path / "tuning.json"
+    f.write_text(json.dumps(tuning))
+    out = tmp_path / "gen"
+    env = dict(os.environ, QX_TUNING_FILE=str(f))
+    subprocess.run([sys.executable, os.path.join(ROOT, "quax_amd", "csrc", "gen_eri.py"), str(out), "1"],
+                   check=True, capture_output=True, env=env)
+    inc = (out / "qx_classes.inc").read_text()
+    classes = re.findall(r"^  \{(\d), (\d), (\d), (\d), qx_launch_e", inc, re.M)
+    names = ["".join(c) for c in classes]
+
+    def table(name):
+        body = inc.split(f"{name}[QX_NUM_CLASSES][2] = {{", 1)[1].split("};", 1)[0]
+        return [tuple(int(x) for x in r) for r in re.findall(r"\{(\d+), (\d+)\}", body)]
+
+    tiles, fp32 = table("qx_tile_default"), table("qx_fp32_default")
+    assert len(tiles) == len(fp32) == len(names)
+    for nm, t, f32 in zip(names, tiles, fp32):
+        assert t == (tuning["tiles"]["jk"].get(nm, 128), tuning["tiles"]["grad"].get(nm, 128)), nm
+        assert f32 == (int(nm in tuning["fp32"]["jk"]), int(nm in tuning["fp32"]["grad"])), nm
