@@ -86,7 +86,8 @@ struct QxEpiDev {
   const double* D;           // density, Cartesian nbf x nbf (JK, GRAD)
   double* J; double* K;      // unsymmetrised J', K' (JK)
   int krows;                 // JK: stage K' rows of a block-uniform bra pair in LDS
-  double* grad;              // 3N accumulators (GRAD)
+  double* grad;              // GRAD: grad_copies x 3N accumulators (copy = block index % grad_copies)
+  int grad_copies;           // GRAD: copies of the 3N accumulators (0 or 1: one)
   const double* tv;          // 3N geometry direction: tangent kernels' seed, JKD weights
   const int* slot_map;       // DFULL: output slice of coordinate y (-1: none); null = coord rule
   int skip_single;           // FULL: leave out quartets with all four centres on one atom
@@ -742,8 +743,7 @@ __device__ __forceinline__ void qx_epi_djk(
     }                                                                                             \
     if (lds) {                                                                                    \
       __syncthreads();                                                                            \
-      for (int i = threadIdx.x; i < ng; i += QX_BLOCK)                                            \
-        if (sg[i] != 0.0) qx_atomic_add(&A_.e.grad[i], sg[i]);                                    \
+      qx_grad_flush(A_, sg, ng);                                                                  \
     }                                                                                             \
   }                                                                                               \
   extern "C" int qx_launcht_##NAME(const QxArgs* a, const QxLaunch* L) {                          \
@@ -761,6 +761,47 @@ __device__ __forceinline__ void qx_epi_djk(
 // parts in the group) and `part` dispatches on blockIdx.y.  This build emits
 // groups of one part.
 #define QX_GRAD_NCOMB 9
+
+// Adds a lane's 12 centre x direction gradient terms gv (zero on inactive
+// lanes) to the block's LDS accumulators sg, or to the global ones when the
+// gradient does not fit in LDS.  A centre whose atom is the same on every
+// active lane of the wavefront (the bra centres when the wave shares a bra
+// pair) is summed across the wave first and added by one lane, so the lanes
+// do not serialise on one LDS address.  Every lane calls it (converged).
+__device__ __forceinline__ void qx_grad_to_block(const QxArgs& A_, double* sg, bool lds, bool active,
+                                                 const int (&at)[4], const double (&gv)[12]) {
+  const bool lead = qx_wave_leader(active);
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const bool uni = qx_wave_uniform(at[c], active);  // wave-uniform value
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+      const int slot = 3 * at[c] + x;
+      if (uni) {
+        const double v = qx_wave_sum(gv[3 * c + x]);
+        if (lead && v != 0.0) {
+          if (lds) atomicAdd(&sg[slot], v);
+          else qx_atomic_add(&A_.e.grad[slot], v);
+        }
+      } else if (active && gv[3 * c + x] != 0.0) {
+        if (lds) atomicAdd(&sg[slot], gv[3 * c + x]);
+        else qx_atomic_add(&A_.e.grad[slot], gv[3 * c + x]);
+      }
+    }
+  }
+}
+
+// Block flush of the LDS accumulators into copy (block index % grad_copies)
+// of the global ones: the blocks of a pass spread their atomics over the
+// copies instead of all meeting on the same 3N addresses (the runtime sums
+// the copies once per pass).
+__device__ __forceinline__ void qx_grad_flush(const QxArgs& A_, const double* sg, int ng) {
+  const int nc = A_.e.grad_copies > 1 ? A_.e.grad_copies : 1;
+  double* g = A_.e.grad + (size_t)(blockIdx.x % nc) * ng;
+  for (int i = threadIdx.x; i < ng; i += blockDim.x)
+    if (sg[i] != 0.0) qx_atomic_add(&g[i], sg[i]);
+}
+
 template <class Part>
 __device__ __forceinline__ void qx_grad_body(const QxArgs& A_, double* sg, Part part) {
   const int ng = 3 * A_.e.natom;
@@ -775,27 +816,23 @@ __device__ __forceinline__ void qx_grad_body(const QxArgs& A_, double* sg, Part 
     for (int i = threadIdx.x; i < ng; i += bs) sg[i] = 0.0;
     __syncthreads();
   }
+  if (active && qx_single_atom(A_, P1, P2)) active = false;
+  double gv[12];
+#pragma unroll
+  for (int k = 0; k < 12; ++k) gv[k] = 0.0;
+  int at[4] = {0, 0, 0, 0};
   if (active) {
-    if (!qx_single_atom(A_, P1, P2)) {
-      double g[QX_GRAD_NCOMB];
+    double g[QX_GRAD_NCOMB];
 #pragma unroll
-      for (int c = 0; c < QX_GRAD_NCOMB; ++c) g[c] = 0.0;
-      part(P1, P2, u1, u2, g);
-      double gv[12];
-      qx_grad_spread<7, 7, 1, QX_GRAD_NCOMB>(8.0 * qx_degeneracy(A_, P1, P2), g, gv);
-      const int at[4] = {A_.pr.atom_a[P1], A_.pr.atom_b[P1], A_.pr.atom_a[P2], A_.pr.atom_b[P2]};
-#pragma unroll
-      for (int k = 0; k < 12; ++k) {
-        const int slot = 3 * at[k / 3] + k % 3;
-        if (lds) atomicAdd(&sg[slot], gv[k]);
-        else qx_atomic_add(&A_.e.grad[slot], gv[k]);
-      }
-    }
+    for (int c = 0; c < QX_GRAD_NCOMB; ++c) g[c] = 0.0;
+    part(P1, P2, u1, u2, g);
+    qx_grad_spread<7, 7, 1, QX_GRAD_NCOMB>(8.0 * qx_degeneracy(A_, P1, P2), g, gv);
+    at[0] = A_.pr.atom_a[P1]; at[1] = A_.pr.atom_b[P1]; at[2] = A_.pr.atom_a[P2]; at[3] = A_.pr.atom_b[P2];
   }
+  qx_grad_to_block(A_, sg, lds, active, at, gv);
   if (lds) {
     __syncthreads();
-    for (int i = threadIdx.x; i < ng; i += bs)
-      if (sg[i] != 0.0) qx_atomic_add(&A_.e.grad[i], sg[i]);
+    qx_grad_flush(A_, sg, ng);
   }
 }
 

@@ -18,6 +18,15 @@ __global__ void k_fill(double* p, long long n, double v) {
     p[i] = v;
 }
 
+// out[i] += sum_c copies[c * n + i]
+__global__ void k_add_copies(const double* copies, int ncopy, int n, double* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double v = 0.0;
+  for (int c = 0; c < ncopy; ++c) v += copies[(size_t)c * n + i];
+  out[i] += v;
+}
+
 __global__ void k_sym_half(const double* in, double* out, int n) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long long)n * n) return;
@@ -468,6 +477,12 @@ int qxk_fill(double* p, long long n, double v, hipStream_t s) {
   if (n <= 0) return 0;
   const long long blocks = std::min<long long>((n + 255) / 256, 1 << 20);
   hipLaunchKernelGGL(k_fill, dim3((unsigned)blocks), dim3(256), 0, s, p, n, v);
+  return (int)hipGetLastError();
+}
+
+int qxk_add_copies(const double* copies, int ncopy, int n, double* out, hipStream_t s) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_add_copies, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, copies, ncopy, n, out);
   return (int)hipGetLastError();
 }
 

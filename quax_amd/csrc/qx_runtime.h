@@ -105,6 +105,9 @@ struct qx_ctx {
   // Cartesian D, symmetrised J and K); ws_ev orders a pass after the previous
   // pass that used them, whatever stream each ran on
   double* ws = nullptr;
+  // QX_GRAD_COPIES copies of the 3N gradient accumulators that the blocks of
+  // an adjoint-gradient pass spread their flushes over (allocated on first use)
+  double* gcopies = nullptr;
   // device address of the file-scope scratch array on this context's device
   // (qx_jk_build_noalloc)
   double* scratch = nullptr;
@@ -147,6 +150,7 @@ struct qx_ctx {
 extern "C" {
 int qxk_fill(double* p, long long n, double v, hipStream_t s);
 int qxk_sym_half(const double* in, double* out, int n, hipStream_t s);    // out = (in + in^T)/2
+int qxk_add_copies(const double* copies, int ncopy, int n, double* out, hipStream_t s);  // out += sum of copies
 int qxk_cart_to_ao_mat(const QxShellsDev* sh, const double* in, double* out, hipStream_t s);   // T X T^T
 int qxk_ao_to_cart_mat(const QxShellsDev* sh, const double* in, double* out, hipStream_t s);   // T^T X T
 int qxk_cart_to_ao_axis(const QxShellsDev* sh, const double* in, double* out, long long outer, int axis_len_in,

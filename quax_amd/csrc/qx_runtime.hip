@@ -515,6 +515,17 @@ void schwarz_and_lists(qx_ctx* c) {
   }
 }
 
+// Copies of the 3N gradient accumulators of an adjoint-gradient pass,
+// cleared on stream s (the pass's blocks flush into copy block % copies; the
+// runtime adds the copies to the caller's gradient after the pass).
+#define QX_GRAD_COPIES 64
+static double* begin_grad_copies(qx_ctx* c, hipStream_t s) {
+  const long long n = (long long)QX_GRAD_COPIES * 3 * c->natom;
+  if (!c->gcopies) c->gcopies = dalloc<double>(c, (size_t)n);
+  check_launch(qxk_fill(c->gcopies, n, 0.0, s), "fill");
+  return c->gcopies;
+}
+
 // Work matrices of a J/K or gradient pass on stream s (see qx_ctx::ws).
 double* begin_ws(qx_ctx* c, hipStream_t s) {
   if (!c->ws) {
@@ -1233,7 +1244,9 @@ int qx_eri_grad(qx_ctx* c, const double* D, double* grad, int precision, double 
       Dcart = w + 2 * n2;
     }
     QxEpiDev e{};
-    e.epi = QX_EPI_GRAD; e.nbf = (int)nc; e.natom = c->natom; e.coord = -1; e.D = Dcart; e.grad = grad;
+    e.epi = QX_EPI_GRAD; e.nbf = (int)nc; e.natom = c->natom; e.coord = -1; e.D = Dcart;
+    e.grad = begin_grad_copies(c, s);
+    e.grad_copies = QX_GRAD_COPIES;
     std::vector<Launch> v;
     if (precision == QX_PREC_MIXED) {
       add_mixed_launches(c, thr, 1, e, v);
@@ -1241,6 +1254,7 @@ int qx_eri_grad(qx_ctx* c, const double* D, double* grad, int precision, double 
       add_deriv_launches(c, c->work, e, 0, v);
     }
     run_launches(c, v, s, "eri_grad");
+    check_launch(qxk_add_copies(c->gcopies, QX_GRAD_COPIES, 3 * c->natom, grad, s), "gradient copies");
     end_ws(c, s);
     return QX_OK;
   });
@@ -1263,10 +1277,13 @@ int qx_eri_hess_dir(qx_ctx* c, const double* D, const double* v, double* out, vo
       Dcart = w + 2 * n2;
     }
     QxEpiDev e{};
-    e.epi = QX_EPI_GRAD; e.nbf = (int)nc; e.natom = c->natom; e.coord = -1; e.D = Dcart; e.grad = out; e.tv = v;
+    e.epi = QX_EPI_GRAD; e.nbf = (int)nc; e.natom = c->natom; e.coord = -1; e.D = Dcart; e.tv = v;
+    e.grad = begin_grad_copies(c, s);
+    e.grad_copies = QX_GRAD_COPIES;
     std::vector<Launch> L;
     add_dert_launches(c, e, L);
     run_launches(c, L, s, "eri_hess_dir");
+    check_launch(qxk_add_copies(c->gcopies, QX_GRAD_COPIES, 3 * c->natom, out, s), "gradient copies");
     end_ws(c, s);
     return QX_OK;
   });

