@@ -274,6 +274,19 @@ const char* qx_scf_last_error(const qx_scf* scf);
  * quax_amd/csrc/tune.py --tiles, from GPU timings). */
 int qx_set_class_tiles(qx_ctx* ctx, int pass, const int* block);
 int qx_get_class_tiles(const qx_ctx* ctx, int pass, int* block);
+/* Per class (flags[qx_num_classes()], NULL = built-in table): whether the
+ * adjoint-gradient kernels sum the terms of centres that are the same atom
+ * on every lane of a wavefront across the wavefront before their LDS
+ * accumulators (cheaper LDS atomics, ~200 cross-lane operations per wave);
+ * the autotuner picks it per class. */
+int qx_set_grad_wave_reduce(qx_ctx* ctx, const int* flags);
+int qx_get_grad_wave_reduce(const qx_ctx* ctx, int* flags);
+/* Per class (copies[qx_num_classes()], 1..64, NULL = built-in table): how
+ * many copies of the 3N gradient accumulators the blocks of its
+ * adjoint-gradient kernels spread their flushes over (block index mod
+ * copies); the copies are summed once per pass. */
+int qx_set_grad_copies(qx_ctx* ctx, const int* copies);
+int qx_get_grad_copies(const qx_ctx* ctx, int* copies);
 
 /* qx_jk_build for a batch context (qx_initialize_fragments) that leaves out
  * the quartets of every fragment f with frag_skip[f] != 0 (device int array,

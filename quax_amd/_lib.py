@@ -77,6 +77,10 @@ SIGNATURES = {
     "qx_get_fp32_classes": (_i, [_vp, _i, _vp]),
     "qx_set_streams": (_i, [_vp, _i]),
     "qx_set_class_tiles": (_i, [_vp, _i, _vp]),
+    "qx_set_grad_wave_reduce": (_i, [_vp, _vp]),
+    "qx_set_grad_copies": (_i, [_vp, _vp]),
+    "qx_get_grad_copies": (_i, [_vp, _vp]),
+    "qx_get_grad_wave_reduce": (_i, [_vp, _vp]),
     "qx_get_class_tiles": (_i, [_vp, _i, _vp]),
     "qx_jk_build_masked": (_i, [_vp, _vp, _vp, _vp, _i, ctypes.c_double, _vp, _vp]),
     "qx_scf_create": (_i, [_vp, _i, _i, _ip, _ip, _ip, _dp, _vp, _vp, _vp, _i, _i, _i, ctypes.c_double, _i, _i, _i,

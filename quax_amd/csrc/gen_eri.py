@@ -69,6 +69,23 @@ if os.path.exists(_BUDGET_FILE):
         _t = json.load(_fh).get("tiles", {})
         TILES = {k: {tuple(int(ch) for ch in c): int(v) for c, v in _t.get(k, {}).items()} for k in ("jk", "grad")}
 
+# classes whose adjoint-gradient kernels sum wave-uniform centres across the
+# wavefront before the LDS accumulators (picked by tune.py --tiles)
+GRAD_WAVE_REDUCE = {(0, 0, 0, 0), (1, 0, 0, 0), (1, 0, 1, 0), (1, 1, 0, 0), (2, 0, 0, 0)}
+if os.path.exists(_BUDGET_FILE):
+    with open(_BUDGET_FILE) as _fh:
+        _g = json.load(_fh).get("grad_wave_reduce")
+        if _g is not None:
+            GRAD_WAVE_REDUCE = {tuple(int(ch) for ch in c) for c in _g}
+
+# copies of the gradient accumulators each class's adjoint blocks flush into
+# (picked by tune.py --tiles; default 64)
+GRAD_COPIES = {}
+if os.path.exists(_BUDGET_FILE):
+    with open(_BUDGET_FILE) as _fh:
+        GRAD_COPIES = {tuple(int(ch) for ch in c): int(v)
+                       for c, v in json.load(_fh).get("grad_copies", {}).items()}
+
 # classes whose mixed-precision passes may use FP32 (J/K, gradient): those
 # whose FP32 kernels timed faster than their FP64 ones (fewer spills)
 FP32_CLASSES = {"jk": set(), "grad": set()}
@@ -762,6 +779,14 @@ def main():
     D.append("static const unsigned char qx_fp32_default[QX_NUM_CLASSES][2] = {")
     for cls in classes:
         D.append(f"  {{{int(cls in FP32_CLASSES['jk'])}, {int(cls in FP32_CLASSES['grad'])}}},")
+    D.append("};")
+    D.append("// per class: the adjoint-gradient kernels reduce wave-uniform centres across the wavefront")
+    D.append("static const unsigned char qx_gwred_default[QX_NUM_CLASSES] = {")
+    D.append("  " + ", ".join(str(int(cls in GRAD_WAVE_REDUCE)) for cls in classes))
+    D.append("};")
+    D.append("// per class: copies of the gradient accumulators its adjoint blocks flush into")
+    D.append("static const int qx_gcopy_default[QX_NUM_CLASSES] = {")
+    D.append("  " + ", ".join(str(GRAD_COPIES.get(cls, 64)) for cls in classes))
     D.append("};")
     D.append("// per class: threads per block of the streaming J/K and gradient kernels")
     D.append("static const int qx_tile_default[QX_NUM_CLASSES][2] = {")

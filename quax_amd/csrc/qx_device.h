@@ -86,13 +86,16 @@ struct QxEpiDev {
   const double* D;           // density, Cartesian nbf x nbf (JK, GRAD)
   double* J; double* K;      // unsymmetrised J', K' (JK)
   int krows;                 // JK: stage K' rows of a block-uniform bra pair in LDS
+  int kcols;                 // JK: width of those LDS rows (widest fragment's function range; 0 = nbf)
+  const int* frag_col0;      // JK: first function of each fragment's range (null: 0); with pair_frag
   double* grad;              // GRAD: grad_copies x 3N accumulators (copy = block index % grad_copies)
   int grad_copies;           // GRAD: copies of the 3N accumulators (0 or 1: one)
+  int grad_wred;             // GRAD: sum wave-uniform centres across the wavefront before LDS
   const double* tv;          // 3N geometry direction: tangent kernels' seed, JKD weights
   const int* slot_map;       // DFULL: output slice of coordinate y (-1: none); null = coord rule
   int skip_single;           // FULL: leave out quartets with all four centres on one atom
   const int* frag_skip;      // JK: per fragment, nonzero = leave its quartets out (null: none)
-  const int* pair_frag;      // JK: fragment of each sorted shell pair (with frag_skip)
+  const int* pair_frag;      // JK: fragment of each sorted shell pair (with frag_skip / frag_col0)
 };
 
 struct QxArgs {
@@ -293,12 +296,14 @@ __device__ __forceinline__ void qx_scatter8(double* out, size_t n, size_t I, siz
 // Flushes per-thread J'/K' partials: J'_ab += 4f sum_cd (ab|cd) D_cd etc.
 // Every lane calls this (converged); inactive lanes hold zeros.
 // sk != null: the whole block shares the bra pair, and K'_ac, K'_ad, K'_bc,
-// K'_bd go to its LDS rows (a: 0..NA-1, b: NA..NA+NB-1; columns are
-// functions) instead of global atomics.
+// K'_bd go to its LDS rows (a: 0..NA-1, b: NA..NA+NB-1; column j of a row is
+// function col0 + j, the bra fragment's range, where every ket of the block
+// lies) instead of global atomics.
 // Only rows a in [A0, A1) of J'_ab, K'_ac and K'_ad can be non-zero (a J/K
 // part covers that range of a components).
 template <int NA, int NB, int NC, int ND, int A0, int A1>
 __device__ __forceinline__ void qx_jk_flush(const QxArgs& A_, int P1, int P2, bool active, double* sk,
+                                            int W, int col0,
                                             const double (&jab)[NA * NB], const double (&jcd)[NC * ND],
                                             const double (&kac)[NA * NC], const double (&kad)[NA * ND],
                                             const double (&kbc)[NB * NC], const double (&kbd)[NB * ND]) {
@@ -327,14 +332,15 @@ __device__ __forceinline__ void qx_jk_flush(const QxArgs& A_, int P1, int P2, bo
 #pragma unroll
   for (int k = 0; k < NC * ND; ++k) qx_atomic_add(&J[(ic + k / ND) * n + id + k % ND], fj * jcd[k]);
   if (sk) {
+    const int jc = ic - col0, jd = id - col0;
 #pragma unroll
-    for (int k = A0 * NC; k < A1 * NC; ++k) atomicAdd(&sk[(k / NC) * n + ic + k % NC], fk * kac[k]);
+    for (int k = A0 * NC; k < A1 * NC; ++k) atomicAdd(&sk[(k / NC) * W + jc + k % NC], fk * kac[k]);
 #pragma unroll
-    for (int k = A0 * ND; k < A1 * ND; ++k) atomicAdd(&sk[(k / ND) * n + id + k % ND], fk * kad[k]);
+    for (int k = A0 * ND; k < A1 * ND; ++k) atomicAdd(&sk[(k / ND) * W + jd + k % ND], fk * kad[k]);
 #pragma unroll
-    for (int k = 0; k < NB * NC; ++k) atomicAdd(&sk[(NA + k / NC) * n + ic + k % NC], fk * kbc[k]);
+    for (int k = 0; k < NB * NC; ++k) atomicAdd(&sk[(NA + k / NC) * W + jc + k % NC], fk * kbc[k]);
 #pragma unroll
-    for (int k = 0; k < NB * ND; ++k) atomicAdd(&sk[(NA + k / ND) * n + id + k % ND], fk * kbd[k]);
+    for (int k = 0; k < NB * ND; ++k) atomicAdd(&sk[(NA + k / ND) * W + jd + k % ND], fk * kbd[k]);
     return;
   }
 #pragma unroll
@@ -624,8 +630,9 @@ __device__ __forceinline__ void qx_epi_djk(
     if (qx_block_idle(A_, active)) return;                                                        \
     if (!active) P1 = P2 = -1;                                                                    \
     const int n_ = A_.e.nbf;                                                                      \
+    const int W_ = A_.e.kcols > 0 ? A_.e.kcols : n_;                                              \
     double* sk = nullptr;                                                                         \
-    int ra = 0, rb = 0;                                                                           \
+    int ra = 0, rb = 0, c0_ = 0;                                                                  \
     if (A_.e.krows) { /* same result in every thread: block-uniform */                            \
       const long long t0 = (long long)blockIdx.x * bs_;                                           \
       const long long t1 = min(t0 + bs_, A_.w.nquart) - 1;                                        \
@@ -636,20 +643,21 @@ __device__ __forceinline__ void qx_epi_djk(
         sk = qx_sk;                                                                               \
         ra = A_.pr.bfa[B0];                                                                       \
         rb = A_.pr.bfb[B0];                                                                       \
-        for (int i = threadIdx.x; i < (NA + NB) * n_; i += bs_) sk[i] = 0.0;                      \
+        if (A_.e.frag_col0) c0_ = A_.e.frag_col0[A_.e.pair_frag[B0]];                             \
+        for (int i = threadIdx.x; i < (NA + NB) * W_; i += bs_) sk[i] = 0.0;                      \
         __syncthreads();                                                                          \
       }                                                                                           \
     }                                                                                             \
     double jab[NA * NB] = {}, jcd[NC * ND] = {}, kac[NA * NC] = {}, kad[NA * ND] = {},            \
            kbc[NB * NC] = {}, kbd[NB * ND] = {};                                                  \
     if (active) qx_jk_##NAME<R>(A_, P1, P2, u1, u2, jab, jcd, kac, kad, kbc, kbd);                \
-    qx_jk_flush<NA, NB, NC, ND, A0, A1>(A_, P1, P2, active, sk, jab, jcd, kac, kad, kbc, kbd);    \
+    qx_jk_flush<NA, NB, NC, ND, A0, A1>(A_, P1, P2, active, sk, W_, c0_, jab, jcd, kac, kad, kbc, kbd); \
     if (sk) {                                                                                     \
       __syncthreads();                                                                            \
-      for (int i = threadIdx.x; i < (NA + NB) * n_; i += bs_) {                                   \
+      for (int i = threadIdx.x; i < (NA + NB) * W_; i += bs_) {                                   \
         const double v = sk[i];                                                                   \
         if (v != 0.0) {                                                                           \
-          const int r = i / n_, col = i - r * n_;                                                 \
+          const int r = i / W_, col = c0_ + (i - r * W_);                                         \
           const int row = r < NA ? ra + r : rb + (r - NA);                                        \
           qx_atomic_add(&A_.e.K[(size_t)row * n_ + col], v);                                      \
         }                                                                                         \
@@ -660,7 +668,7 @@ __device__ __forceinline__ void qx_epi_djk(
     if (a->w.nquart <= 0) return 0;                                                               \
     if (a->e.epi != QX_EPI_JK) return -1;                                                         \
     QxArgs b = *a;                                                                                \
-    const size_t rows_bytes = (size_t)(NA + NB) * (size_t)a->e.nbf * sizeof(double);              \
+    const size_t rows_bytes = (size_t)(NA + NB) * (size_t)(a->e.kcols > 0 ? a->e.kcols : a->e.nbf) * sizeof(double); \
     b.e.krows = a->e.krows && ((A1 - A0) * (NC + ND) + NB * NC + NB * ND) >= QX_JK_ROWS_MIN &&    \
                 rows_bytes <= QX_JK_LDS_MAX;                                                      \
     const size_t sh = b.e.krows ? rows_bytes : 0;                                                 \
@@ -764,12 +772,24 @@ __device__ __forceinline__ void qx_epi_djk(
 
 // Adds a lane's 12 centre x direction gradient terms gv (zero on inactive
 // lanes) to the block's LDS accumulators sg, or to the global ones when the
-// gradient does not fit in LDS.  A centre whose atom is the same on every
-// active lane of the wavefront (the bra centres when the wave shares a bra
-// pair) is summed across the wave first and added by one lane, so the lanes
-// do not serialise on one LDS address.  Every lane calls it (converged).
+// gradient does not fit in LDS.  With grad_wred (a per-class choice), a
+// centre whose atom is the same on every active lane of the wavefront (the
+// bra centres when the wave shares a bra pair) is summed across the wave
+// first and added by one lane, so the lanes do not serialise on one LDS
+// address; that costs ~200 cross-lane operations per wave, which only the
+// cheapest classes repay.  Every lane calls it (converged).
 __device__ __forceinline__ void qx_grad_to_block(const QxArgs& A_, double* sg, bool lds, bool active,
                                                  const int (&at)[4], const double (&gv)[12]) {
+  if (!A_.e.grad_wred) {  // launch-uniform
+    if (!active) return;
+#pragma unroll
+    for (int k = 0; k < 12; ++k) {
+      const int slot = 3 * at[k / 3] + k % 3;
+      if (lds) atomicAdd(&sg[slot], gv[k]);
+      else qx_atomic_add(&A_.e.grad[slot], gv[k]);
+    }
+    return;
+  }
   const bool lead = qx_wave_leader(active);
 #pragma unroll
   for (int c = 0; c < 4; ++c) {

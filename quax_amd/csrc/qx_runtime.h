@@ -121,7 +121,17 @@ struct qx_ctx {
   // threads per block of the streaming kernels, per pass (0 J/K, 1 adjoint
   // gradient) and class (the tuned quartet tiles)
   int tile[2][QX_NUM_CLASSES_MAX] = {};
+  // per class: the adjoint-gradient kernels sum wave-uniform centres across
+  // the wavefront before their LDS accumulators
+  unsigned char gwred[QX_NUM_CLASSES_MAX] = {};
+  // per class: copies of the gradient accumulators its blocks spread their
+  // flushes over (1..QX_GRAD_COPIES)
+  int gcopy[QX_NUM_CLASSES_MAX] = {};
   int* d_pair_frag = nullptr;            // fragment of each sorted pair (fragmented contexts)
+  // fragmented contexts: first Cartesian function of each fragment id's range
+  // and the widest range (the width of the J/K kernels' LDS K' rows)
+  int* d_frag_col0 = nullptr;
+  int kcols = 0;
   // mixed precision: per class and pass (0 J/K, 1 gradient) whether quartets
   // below the threshold run in FP32 (classes whose FP32 kernels are faster)
   unsigned char fp32_ok[2][QX_NUM_CLASSES_MAX] = {};

@@ -501,6 +501,22 @@ void schwarz_and_lists(qx_ctx* c) {
   if (c->fragmented) {
     c->d_kstart = dupload(c, allkst);
     c->d_pair_frag = dupload(c, c->h_pair_frag);
+    // Cartesian function range of each fragment id (all kets of a bra pair
+    // lie in its fragment, so a block's K' rows span at most this range)
+    int fmax = 0;
+    for (int f : c->frag) fmax = std::max(fmax, f);
+    std::vector<int> lo(fmax + 1, INT32_MAX), hi(fmax + 1, -1);
+    for (int sh = 0; sh < c->nshell; ++sh) {
+      const int f = c->frag[c->atom[sh]];
+      lo[f] = std::min(lo[f], c->bfc[sh]);
+      hi[f] = std::max(hi[f], c->bfc[sh] + ncart(c->l[sh]));
+    }
+    c->kcols = 0;
+    for (int f = 0; f <= fmax; ++f) {
+      if (hi[f] < 0) lo[f] = 0;
+      else c->kcols = std::max(c->kcols, hi[f] - lo[f]);
+    }
+    c->d_frag_col0 = dupload(c, lo);
   }
   c->h_prefix = allpre;
   c->h_kstart = allkst;
@@ -512,6 +528,23 @@ void schwarz_and_lists(qx_ctx* c) {
     auto& wh = c->work[k];
     wh.w.prefix = c->d_prefix + wh.prefix_off;
     wh.w.kstart = c->fragmented ? c->d_kstart + kst_off[k] : nullptr;
+  }
+}
+
+#define QX_JK_ROWS_NMAX 2048
+
+// K' rows of a J/K pass in LDS (each class kernel still decides whether its
+// rows fit): over the bra fragment's function range in a fragmented context,
+// over all functions otherwise; none when that width exceeds
+// QX_JK_ROWS_NMAX, where zeroing and scanning a block's rows would cost more
+// than its quartets.
+static void set_krows(const qx_ctx* c, QxEpiDev& e) {
+  const int width = c->fragmented ? c->kcols : c->ncart;
+  e.krows = width <= QX_JK_ROWS_NMAX ? 1 : 0;
+  if (c->fragmented) {
+    e.kcols = c->kcols;
+    e.frag_col0 = c->d_frag_col0;
+    e.pair_frag = c->d_pair_frag;
   }
 }
 
@@ -706,6 +739,8 @@ static int init_impl(int natom, const double* xyz_bohr, const double* nuclear_ch
         for (int pass = 0; pass < 2; ++pass) {
           c->fp32_ok[pass][k] = qx_fp32_default[k][pass];
           c->tile[pass][k] = qx_tile_default[k][pass];
+          c->gwred[k] = qx_gwred_default[k];
+          c->gcopy[k] = qx_gcopy_default[k];
         }
       init_body(c, natom, xyz_bohr, nuclear_charges, nshell, shell_l, shell_atom, shell_nprim, exps, coefs, pure,
                 ints_tol, device, atom_fragment);
@@ -915,6 +950,10 @@ static void add_deriv_list(qx_ctx* c, const QxWorkHost& wh, const QxEpiDev& e, i
     QxArgs a;
     a.pr = c->pr; a.pp = c->pp; a.w = wh.w; a.e = e; a.boys = c->d_boys;
     a.w.sel = sel;
+    if (grad) {
+      a.e.grad_wred = c->gwred[wh.cls];
+      a.e.grad_copies = std::min(std::max(c->gcopy[wh.cls], 1), a.e.grad_copies);
+    }
     const int slot = QX_SLOT_DERIV(wh.cls, k, fp32 ? 1 : 0);
     auto fn = grad ? ent.grad[k] : ent.deri[k];
     const double fl = grad ? ent.group_flops[k] : ent.part_flops[k];
@@ -1148,7 +1187,7 @@ int qx_jk_build_masked(qx_ctx* c, const double* D, double* J, double* K, int pre
     check_launch(qxk_fill(Kp, (long long)n2, 0.0, s), "fill");
     QxEpiDev e{};
     e.epi = QX_EPI_JK; e.nbf = (int)nc; e.natom = c->natom; e.coord = -1; e.D = Dcart; e.J = Jp; e.K = Kp;
-    e.krows = 1;  // each class kernel decides whether its K' rows fit in LDS
+    set_krows(c, e);
     if (frag_skip) {
       e.frag_skip = frag_skip;
       e.pair_frag = c->d_pair_frag;
@@ -1202,7 +1241,7 @@ int qx_jk_build_noalloc(qx_ctx* c, const double* D, double* J, double* K, void* 
     check_launch(qxk_fill(Kp, (long long)n2, 0.0, s), "fill");
     QxEpiDev e{};
     e.epi = QX_EPI_JK; e.nbf = (int)nc; e.natom = c->natom; e.coord = -1; e.D = Dcart; e.J = Jp; e.K = Kp;
-    e.krows = 1;
+    set_krows(c, e);
     for (const auto& wh : c->work) {
       if (wh.w.nquart <= 0) continue;
       const QxClassEntry& ent = qx_class_table[wh.cls];
@@ -1532,6 +1571,33 @@ int qx_set_class_tiles(qx_ctx* c, int pass, const int* block) {
     for (int k = 0; k < QX_NUM_CLASSES; ++k)
       if (block[k] < 64 || block[k] > QX_BLOCK_MAX || block[k] % 64) return QX_EINVAL;
   for (int k = 0; k < QX_NUM_CLASSES; ++k) c->tile[pass][k] = block ? block[k] : qx_tile_default[k][pass];
+  return QX_OK;
+}
+
+int qx_set_grad_wave_reduce(qx_ctx* c, const int* flags) {
+  if (!c) return QX_EINVAL;
+  for (int k = 0; k < QX_NUM_CLASSES; ++k) c->gwred[k] = flags ? (flags[k] != 0) : qx_gwred_default[k];
+  return QX_OK;
+}
+
+int qx_set_grad_copies(qx_ctx* c, const int* copies) {
+  if (!c) return QX_EINVAL;
+  if (copies)
+    for (int k = 0; k < QX_NUM_CLASSES; ++k)
+      if (copies[k] < 1 || copies[k] > QX_GRAD_COPIES) return QX_EINVAL;
+  for (int k = 0; k < QX_NUM_CLASSES; ++k) c->gcopy[k] = copies ? copies[k] : qx_gcopy_default[k];
+  return QX_OK;
+}
+
+int qx_get_grad_copies(const qx_ctx* c, int* copies) {
+  if (!c || !copies) return QX_EINVAL;
+  for (int k = 0; k < QX_NUM_CLASSES; ++k) copies[k] = c->gcopy[k];
+  return QX_OK;
+}
+
+int qx_get_grad_wave_reduce(const qx_ctx* c, int* flags) {
+  if (!c || !flags) return QX_EINVAL;
+  for (int k = 0; k < QX_NUM_CLASSES; ++k) flags[k] = c->gwred[k];
   return QX_OK;
 }
 

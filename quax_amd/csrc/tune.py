@@ -4,9 +4,11 @@
 threads per workgroup of its streaming J/K and adjoint-gradient kernels,
 i.e. how many primitive quartets a workgroup evaluates and how many share its
 LDS K' rows -- out of 64/128/192/256, timed per class on one stream, and
-the classes whose FP32 kernels beat their FP64 ones by 8% or more (the
-mixed-precision policy).  Writes the "tiles" and "fp32" keys that gen_eri.py
-compiles into the default tables (qx_set_class_tiles / qx_set_fp32_classes
+the classes whose adjoint-gradient kernels gain from summing wave-uniform
+terms across the wavefront, and the classes whose FP32 kernels beat their
+FP64 ones by 8% or more (the mixed-precision policy).  Writes the "tiles",
+"grad_wave_reduce" and "fp32" keys that gen_eri.py compiles into the default
+tables (qx_set_class_tiles / qx_set_grad_wave_reduce / qx_set_fp32_classes
 override them at run time):
     python -m quax_amd.csrc.tune --tiles --merge quax_amd/csrc/deriv_budget.json --out tiles.json
 
@@ -144,6 +146,35 @@ def tune_tiles(workload, steps, seed=0):
               + f"  -> {best}", flush=True)
     for p, key in ((0, "jk"), (1, "grad")):
         eng.set_class_tiles(p, [tiles[key].get(names[c], 128) for c in range(nc)])
+    # copies of the gradient accumulators the blocks flush into
+    gcp = {}
+    for cp in (1, 8, 64):
+        eng.set_grad_copies([cp] * nc)
+        gcp[cp] = _pass_times(eng, D, steps)
+    copies = {}
+    for (cls, p) in sorted(gcp[64]):
+        if p != 1:
+            continue
+        times = {cp: gcp[cp].get((cls, p), 1e300) for cp in gcp}
+        best = min(times, key=times.get)
+        if times[64] <= 1.02 * times[best]:
+            best = 64
+        copies[cls] = best
+        table[f"{cls}/grad_copies"] = times
+    eng.set_grad_copies([copies.get(names[c], 64) for c in range(nc)])
+    print(f"[tune] gradient copies: {copies}", flush=True)
+    # wave reduction of the gradient terms: every class with and without it
+    wred = {}
+    for on in (False, True):
+        eng.set_grad_wave_reduce([on] * nc)
+        wred[on] = _pass_times(eng, D, steps)
+    gwr = sorted(cls for (cls, p), t in wred[True].items()
+                 if p == 1 and t <= 0.98 * wred[False].get((cls, p), 1e300))
+    eng.set_grad_wave_reduce([names[c] in gwr for c in range(nc)])
+    print(f"[tune] wave-reduced gradient terms: {gwr}", flush=True)
+    for (cls, p) in sorted(wred[True]):
+        if p == 1:
+            table[f"{cls}/grad_wave_reduce"] = {"off": wred[False].get((cls, p)), "on": wred[True][(cls, p)]}
     # FP32 policy: every quartet of every class in FP32 against FP64, at the chosen tiles
     t64 = _pass_times(eng, D, steps)
     for p in (0, 1):
@@ -177,6 +208,7 @@ def tune_tiles(workload, steps, seed=0):
         print(f"[tune] four streams, {label} tiles: {res[label]:.2f} ms per step", flush=True)
     eng.close()
     return {"tiles": {k: {c: b for c, b in v.items() if b != 128} for k, v in tiles.items()}, "fp32": fp32,
+            "grad_wave_reduce": gwr, "grad_copies": {c: n for c, n in copies.items() if n != 64},
             "ms_per_step_tiles": {"per_class_one_stream": table, "four_streams": res}}
 
 

@@ -158,6 +158,40 @@ class EriEngine:
         check(self.L.qx_set_class_tiles(self.ctx, int(pass_), f.ctypes.data_as(ctypes.c_void_p)), self.ctx,
               "set_class_tiles")
 
+    def grad_wave_reduce(self):
+        """Per kernel class, whether the adjoint-gradient kernels reduce
+        wave-uniform centres across the wavefront."""
+        f = np.zeros(self.L.qx_num_classes(), dtype=np.int32)
+        check(self.L.qx_get_grad_wave_reduce(self.ctx, f.ctypes.data_as(ctypes.c_void_p)), self.ctx)
+        return f.astype(bool)
+
+    def set_grad_wave_reduce(self, flags=None):
+        """Sets it per class (None: the built-in tuned table)."""
+        if flags is None:
+            check(self.L.qx_set_grad_wave_reduce(self.ctx, None), self.ctx)
+            return
+        f = np.ascontiguousarray(np.asarray(flags, dtype=np.int32))
+        if f.shape != (self.L.qx_num_classes(),):
+            raise ValueError("one flag per kernel class")
+        check(self.L.qx_set_grad_wave_reduce(self.ctx, f.ctypes.data_as(ctypes.c_void_p)), self.ctx)
+
+    def grad_copies(self):
+        """Per kernel class, copies of the gradient accumulators its adjoint
+        blocks flush into."""
+        f = np.zeros(self.L.qx_num_classes(), dtype=np.int32)
+        check(self.L.qx_get_grad_copies(self.ctx, f.ctypes.data_as(ctypes.c_void_p)), self.ctx)
+        return f
+
+    def set_grad_copies(self, copies=None):
+        """Sets them per class (1..64; None: the built-in tuned table)."""
+        if copies is None:
+            check(self.L.qx_set_grad_copies(self.ctx, None), self.ctx)
+            return
+        f = np.ascontiguousarray(np.asarray(copies, dtype=np.int32))
+        if f.shape != (self.L.qx_num_classes(),):
+            raise ValueError("one copy count per kernel class")
+        check(self.L.qx_set_grad_copies(self.ctx, f.ctypes.data_as(ctypes.c_void_p)), self.ctx, "set_grad_copies")
+
     def set_fp32_classes(self, pass_: int, flags=None):
         """Sets the per-class FP32 policy of a mixed pass (None: built-in default)."""
         if flags is None:

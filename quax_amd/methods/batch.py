@@ -71,9 +71,12 @@ def batched_rhf(molecules, basis_name="even-tempered", pure=True, ints_tol=1e-12
         Enuc.append(nuclear_repulsion(m.geometry(), Z))
     blk = [slice(int(off[k]), int(off[k + 1])) for k in range(B)]
     if graph:
+        import time
+        t0 = time.perf_counter()
         E, D, C_all, eps_all, iters, done, stalled = _graph_scf(eng, H, S, A, ndocc, Enuc, off, blk, ntot, conv,
                                                                 maxit, diis_space, prec, fp32_threshold,
                                                                 purify_iters, dev)
+        timing = {"graph_s": time.perf_counter() - t0, "stalled": int(stalled.sum())}
         if stalled.any():
             # the last iterations of molecules whose purified densities hit
             # their rounding floor: eigh-based, from the graph's densities
@@ -85,8 +88,9 @@ def batched_rhf(molecules, basis_name="even-tempered", pure=True, ints_tol=1e-12
                 D[blk[k], blk[k]] = D2[blk[k], blk[k]]
                 iters[k] += it2[k]
                 done[k] = done2[k]
+        timing["scf_s"] = time.perf_counter() - t0
         return _finish(eng, one, molecules, D, C_all, eps_all, blk, aof, ndocc, E, iters, done, nb, gradient, prec,
-                       fp32_threshold)
+                       fp32_threshold, timing)
     E, D, C_all, eps_all, iters, done = _eager_scf(eng, H, S, A, ndocc, Enuc, blk, nb, ntot, conv, maxit, diis_space,
                                                    prec, fp32_threshold, dev)
     return _finish(eng, one, molecules, D, C_all, eps_all, blk, aof, ndocc, E, iters, done, nb, gradient, prec,
@@ -133,7 +137,9 @@ def _eager_scf(eng, H, S, A, ndocc, Enuc, blk, nb, ntot, conv, maxit, diis_space
         return torch.tensordot(c[:m_], torch.stack(diis_F[k]), dims=1)
 
     for it in range(maxit + 1):
-        J, K = eng.jk(D, precision=prec, fp32_threshold=fp32_threshold)
+        # molecules already done are masked out of the J/K pass
+        J, K = eng.jk(D, precision=prec, fp32_threshold=fp32_threshold,
+                      frag_skip=done.astype(np.int32) if done.any() else None)
         dE = np.zeros(B)
         drms = torch.zeros(B, dtype=dt, device=dev)
         Enew = torch.zeros(B, dtype=dt, device=dev)
@@ -206,7 +212,7 @@ def _graph_scf(eng, H, S, A, ndocc, Enuc, off, blk, ntot, conv, maxit, diis_spac
 
 
 def _finish(eng, one, molecules, D, C_all, eps_all, blk, aof, ndocc, E, iters, done, nb, gradient, prec,
-            fp32_threshold):
+            fp32_threshold, timing=None):
     grads = None
     if gradient:
         g2 = eng.eri_grad(D, precision=prec, fp32_threshold=fp32_threshold).reshape(-1, 3)
@@ -223,4 +229,7 @@ def _finish(eng, one, molecules, D, C_all, eps_all, blk, aof, ndocc, E, iters, d
     for e1 in one:
         e1.close()
     eng.close()
-    return list(E), grads, {"iterations": iters.tolist(), "converged": done.tolist(), "nbf": nb}
+    aux = {"iterations": iters.tolist(), "converged": done.tolist(), "nbf": nb}
+    if timing:
+        aux.update(timing)
+    return list(E), grads, aux

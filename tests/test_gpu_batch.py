@@ -154,6 +154,14 @@ def test_class_tiles_do_not_change_results(gpu):
         for p in (0, 1):
             eng.set_class_tiles(p, None)
             assert (eng.class_tiles(p) == default[p]).all()
+        # the per-class wave reduction of the gradient terms changes no result
+        wdef = eng.grad_wave_reduce()
+        for flags in (np.ones(nc, dtype=bool), np.zeros(nc, dtype=bool)):
+            eng.set_grad_wave_reduce(flags)
+            assert (eng.grad_wave_reduce() == flags).all()
+            assert np.abs(eng.eri_grad(D).cpu().numpy() - g).max() < 1e-12
+        eng.set_grad_wave_reduce(None)
+        assert (eng.grad_wave_reduce() == wdef).all()
         for bad in ([32] * nc, [320] * nc, [100] * nc):
             with pytest.raises(QuaxNativeError):
                 eng.set_class_tiles(0, bad)

@@ -256,7 +256,7 @@ def test_tuned_tables_reach_the_generated_class_table(tmp_path):
     import subprocess
     import sys
     tuning = {"tiles": {"jk": {"1110": 64}, "grad": {"1111": 256, "1000": 192}},
-              "fp32": {"jk": ["1111"], "grad": ["1110", "1111"]}}
+              "fp32": {"jk": ["1111"], "grad": ["1110", "1111"]}, "grad_wave_reduce": ["1100", "0000"]}
     f = tmp_path / "tuning.json"
     f.write_text(json.dumps(tuning))
     out = tmp_path / "gen"
@@ -273,6 +273,8 @@ def test_tuned_tables_reach_the_generated_class_table(tmp_path):
 
     tiles, fp32 = table("qx_tile_default"), table("qx_fp32_default")
     assert len(tiles) == len(fp32) == len(names)
+    gw = [int(x) for x in inc.split("qx_gwred_default[QX_NUM_CLASSES] = {", 1)[1].split("};", 1)[0].split(",")]
+    assert gw == [int(nm in tuning["grad_wave_reduce"]) for nm in names]
     for nm, t, f32 in zip(names, tiles, fp32):
         assert t == (tuning["tiles"]["jk"].get(nm, 128), tuning["tiles"]["grad"].get(nm, 128)), nm
         assert f32 == (int(nm in tuning["fp32"]["jk"]), int(nm in tuning["fp32"]["grad"])), nm
