@@ -345,8 +345,8 @@ def bench_batch(args):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     # per-molecule check against single-molecule eager runs (the first
-    # --check molecules); molecules whose eager SCF does not settle within 60
-    # iterations have several stationary states that DIIS reaches along
+    # --check molecules); molecules that either SCF needs more than 40
+    # iterations for have several stationary states that DIIS reaches along
     # rounding-dependent paths, so they are counted but not compared
     errs, unsettled = [], 0
     t1 = time.perf_counter()
@@ -358,7 +358,7 @@ def bench_batch(args):
                                                                         "conv": 1e-10, "maxit": 200},
                                                return_aux_data=True)
             gk = rhf_gradient(e1, auxk)
-        if auxk["iterations"] > 60 or auxk["drms"] > 1e-10 or not aux["converged"][k]:
+        if auxk["iterations"] > 40 or aux["iterations"][k] > 40 or auxk["drms"] > 1e-10 or not aux["converged"][k]:
             unsettled += 1
             continue
         errs.append((abs(E[k] - Ek), float(np.abs(G[k] - gk).max())))

@@ -171,12 +171,14 @@ def test_batched_graph_scf_seeded_molecules(gpu):
     """Seeded batch molecules (3-12 atoms, C/H/O) through the graph SCF
     (converged molecules masked out of J/K) against each molecule alone
     through the eager driver: energies 1e-9 Eh, gradients 1e-7.  Only
-    molecules whose eager SCF converges within 60 iterations take part: on
-    some synthetic geometries DIIS from the zero guess wanders for 100-200
+    molecules that both drivers converge within 40 iterations are compared:
+    on some synthetic geometries DIIS from the zero guess wanders for 50-200
     iterations and, depending on rounding-level differences along the way,
     ends in different stationary states (an H-C-H fragment of the set reaches
-    -37.0701200 Eh in numpy and in the eager driver, and a state 1.0e-3 Eh
-    lower in the graph SCF), so those molecules compare no implementation."""
+    -37.0701200 Eh in numpy and in the eager driver and a state 1.0e-3 Eh
+    lower in the graph SCF; a 3-atom one converged in ~60 iterations to
+    states 0.037 Eh apart on one box and to the same state on another), so
+    those molecules compare no implementation."""
     from quax_amd.engine import EriEngine
     from quax_amd.methods.batch import batched_rhf
     from quax_amd.methods.hartree_fock import restricted_hartree_fock, rhf_gradient
@@ -186,7 +188,7 @@ def test_batched_graph_scf_seeded_molecules(gpu):
         b = BasisSet.build(m, "even-tempered", pure=True)
         with EriEngine(m, b, ints_tol=1e-12) as e1:
             Ek, auxk = restricted_hartree_fock(e1, m.nelectron(), opts, return_aux_data=True)
-            if auxk["drms"] > 1e-10 or abs(auxk["dE"]) > 1e-10 or auxk["iterations"] > 60:
+            if auxk["drms"] > 1e-10 or abs(auxk["dE"]) > 1e-10 or auxk["iterations"] > 40:
                 print(f"  {m.natom()} atoms: eager SCF took {auxk['iterations']} iterations "
                       f"(RMS error {auxk['drms']:.1e}), left out", flush=True)
                 continue
@@ -196,8 +198,15 @@ def test_batched_graph_scf_seeded_molecules(gpu):
     E, G, aux = batched_rhf(mols, "even-tempered", pure=True, ints_tol=1e-12, conv=1e-10, graph=True)
     print(f"  graph batch: iterations {aux['iterations']}, converged {aux['converged']}", flush=True)
     assert all(aux["converged"])
+    compared = 0
     for k, (m, Ek, gk) in enumerate(ref):
+        if aux["iterations"][k] > 40:
+            print(f"  molecule {k} ({m.natom()} atoms): graph SCF took {aux['iterations'][k]} iterations, "
+                  f"not compared", flush=True)
+            continue
+        compared += 1
         print(f"  molecule {k} ({m.natom()} atoms): dE {E[k] - Ek:.1e}, max dg {np.abs(G[k] - gk).max():.1e}",
               flush=True)
         assert abs(E[k] - Ek) < 1e-9
         assert np.abs(G[k] - gk).max() < 1e-7
+    assert compared >= 6
