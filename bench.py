@@ -333,15 +333,16 @@ def bench_batch(args):
     E, G, aux = [], [], {"iterations": [], "converged": [], "nbf": []}
     for c0 in range(0, len(mols), args.chunk):
         Ec, Gc, ac = batched_rhf(mols[c0:c0 + args.chunk], "even-tempered", pure=True, ints_tol=1e-12, conv=1e-10,
-                                 graph=graph)
+                                 maxit=args.maxit, graph=graph)
         E += Ec
         G += Gc
         for k in ("iterations", "converged", "nbf"):
             aux[k] += ac[k]
         log(f"[bench] batch molecules {c0}-{c0 + len(Ec) - 1}: {time.perf_counter() - t0:.1f} s so far, "
             f"iterations max {max(ac['iterations'])}, converged {sum(ac['converged'])}/{len(Ec)}"
-            + (f", graph {ac['graph_s']:.1f} s, SCF {ac['scf_s']:.1f} s, {ac['stalled']} finished eagerly"
-               if "graph_s" in ac else ""))
+            + f"; setup {ac['setup_s']:.1f} s, SCF {ac['scf_s']:.1f} s"
+            + (f" (graph {ac['graph_s']:.1f} s, {ac['stalled']} finished eagerly)" if "graph_s" in ac else "")
+            + f", gradient {ac['finish_s']:.1f} s (ERI part {ac.get('eri_grad_s', 0.0):.1f} s)")
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     # per-molecule check against single-molecule eager runs (the first
@@ -355,7 +356,7 @@ def bench_batch(args):
         b = BasisSet.build(mols[k], "even-tempered", pure=True)
         with EriEngine(mols[k], b, ints_tol=1e-12) as e1:
             Ek, auxk = restricted_hartree_fock(e1, mols[k].nelectron(), {"diis": True, "spectral_shift": False,
-                                                                        "conv": 1e-10, "maxit": 200},
+                                                                        "conv": 1e-10, "maxit": args.maxit},
                                                return_aux_data=True)
             gk = rhf_gradient(e1, auxk)
         if auxk["iterations"] > 40 or aux["iterations"][k] > 40 or auxk["drms"] > 1e-10 or not aux["converged"][k]:
@@ -369,11 +370,12 @@ def bench_batch(args):
            "config": {"workload": f"first {len(mols)} of 256 seeded molecules (3-12 atoms, C/H/O 0.4/0.4/0.2), "
                                   f"even-tempered spherical basis, {sum(aux['nbf'])} AO total, Schwarz 1e-12, "
                                   "conv 1e-10",
-                      "chunk": args.chunk, "graph_scf": graph,
+                      "chunk": args.chunk, "graph_scf": graph, "maxit": args.maxit,
                       "wall_s": wall, "iterations_max": max(aux["iterations"]),
                       "all_converged": all(aux["converged"]),
                       "converged": int(sum(aux["converged"])),
                       "single_molecule_s_each": single,
+                      "speedup_vs_single": len(mols) / wall * single,
                       "checked_vs_single": len(errs), "unsettled_in_check": unsettled,
                       "max_energy_error_vs_single": max((e for e, _ in errs), default=None),
                       "max_grad_error_vs_single": max((g for _, g in errs), default=None)}}
@@ -392,6 +394,8 @@ def main():
     ap.add_argument("--chunk", type=int, default=32, help="batch workload: molecules per context and graph")
     ap.add_argument("--no-graph", action="store_true", help="batch workload: host-driven lock-step SCF")
     ap.add_argument("--check", type=int, default=8, help="batch workload: molecules checked against single runs")
+    ap.add_argument("--maxit", type=int, default=100,
+                    help="batch workload: SCF iteration cap (100 is the reference's default, core.py:34)")
     ap.add_argument("--fp32-threshold", type=float, default=1e-8,
                     help="Schwarz bound below which the mixed runs use FP32 quartets")
     ap.add_argument("--streams", type=int, default=4, help="streams per pass (1 = every kernel timed alone)")

@@ -287,6 +287,15 @@ int qx_get_grad_wave_reduce(const qx_ctx* ctx, int* flags);
  * copies); the copies are summed once per pass. */
 int qx_set_grad_copies(qx_ctx* ctx, const int* copies);
 int qx_get_grad_copies(const qx_ctx* ctx, int* copies);
+/* Widest atom window (0..256 atoms; 0 = 256) of the adjoint-gradient
+ * kernels' LDS accumulators.  A block accumulates its quartets' gradient
+ * terms in LDS over the atoms from the lowest to the highest it touches when
+ * that window fits, else adds them to global memory; in a batch context a
+ * block's quartets lie within one or two molecules, so the window stays
+ * narrow however many atoms the batch holds.  A narrower window only moves
+ * blocks to the global path (used by the tests to exercise it). */
+int qx_set_grad_window(qx_ctx* ctx, int atoms);
+int qx_get_grad_window(const qx_ctx* ctx, int* atoms);
 
 /* qx_jk_build for a batch context (qx_initialize_fragments) that leaves out
  * the quartets of every fragment f with frag_skip[f] != 0 (device int array,

@@ -91,6 +91,7 @@ struct QxEpiDev {
   double* grad;              // GRAD: grad_copies x 3N accumulators (copy = block index % grad_copies)
   int grad_copies;           // GRAD: copies of the 3N accumulators (0 or 1: one)
   int grad_wred;             // GRAD: sum wave-uniform centres across the wavefront before LDS
+  int grad_win;              // GRAD: widest LDS atom window in doubles (0 or >= QX_GRAD_LDS: QX_GRAD_LDS)
   const double* tv;          // 3N geometry direction: tangent kernels' seed, JKD weights
   const int* slot_map;       // DFULL: output slice of coordinate y (-1: none); null = coord rule
   int skip_single;           // FULL: leave out quartets with all four centres on one atom
@@ -583,7 +584,7 @@ __device__ __forceinline__ void qx_epi_djk(
 // ---- kernel definitions (instantiated once per generated file) --------
 #define QX_GRID(nq) dim3((unsigned)(((nq) + QX_BLOCK - 1) / QX_BLOCK))
 
-#define QX_GRAD_LDS 768  // block-level gradient accumulators (3 * atoms)
+#define QX_GRAD_LDS 768  // block-level gradient accumulators (3 x the atoms of a block's window)
 
 // Array-form ERI kernel: store / full-tensor scatter / Schwarz epilogues.
 #define QX_DEFINE_ERI_KERNEL(NAME, N, NA, NB, NC, ND)                                             \
@@ -721,37 +722,37 @@ __device__ __forceinline__ void qx_epi_djk(
     using Shape = QxPartShape<NA, NB, NC, ND, A0, A1, B0, B1, C0, C1>;                            \
     static_assert(NCOMB * Shape::NEL * 8 <= 16384,                                                \
                   "per-lane derivative block must stay small (scratch is allocated per lane)");    \
-    const bool grad = A_.e.epi == QX_EPI_GRAD;                                                    \
-    const int ng = 3 * A_.e.natom;                                                                \
-    const bool lds = grad && ng <= QX_GRAD_LDS;                                                   \
+    const bool grad = A_.e.epi == QX_EPI_GRAD; /* launch-uniform */                               \
+    const long long t = (long long)blockIdx.x * QX_BLOCK + threadIdx.x;                          \
+    int u1 = 0, u2 = 0, P1 = -1, P2 = -1;                                                         \
+    bool active = t < A_.w.nquart;                                                                \
+    if (active) active = qx_locate(A_, t, u1, u2, P1, P2) && !qx_single_atom(A_, P1, P2);         \
+    int lo = 0, w = 0;                                                                            \
+    const bool lds = grad && qx_grad_window(A_, active, P1, P2, lo, w);                           \
     if (lds) {                                                                                    \
-      for (int i = threadIdx.x; i < ng; i += QX_BLOCK) sg[i] = 0.0;                               \
+      for (int i = threadIdx.x; i < w; i += QX_BLOCK) sg[i] = 0.0;                                \
       __syncthreads();                                                                            \
     }                                                                                             \
-    const long long t = (long long)blockIdx.x * QX_BLOCK + threadIdx.x;                          \
-    if (t < A_.w.nquart) {                                                                        \
-      int u1, u2, P1, P2;                                                                         \
-      qx_locate(A_, t, u1, u2, P1, P2);                                                           \
-      if (!qx_single_atom(A_, P1, P2)) {                                                          \
-        double dv[NCOMB * Shape::NEL];                                                            \
-        qx_core_##NAME(A_, P1, P2, u1, u2, dv);                                                   \
-        if (grad) {                                                                               \
-          double gv[12];                                                                          \
-          qx_epi_dgrad<NA, NB, NC, ND, CMASK, XMASK, DINV, NCOMB, A0, A1, B0, B1, C0, C1>(A_, P1, P2, dv, gv); \
-          const int at[4] = {A_.pr.atom_a[P1], A_.pr.atom_b[P1], A_.pr.atom_a[P2], A_.pr.atom_b[P2]}; \
-          for (int k = 0; k < 12; ++k) {                                                          \
-            const int slot = 3 * at[k / 3] + k % 3;                                               \
-            if (lds) atomicAdd(&sg[slot], gv[k]);                                                 \
-            else qx_atomic_add(&A_.e.grad[slot], gv[k]);                                          \
-          }                                                                                       \
-        } else {                                                                                  \
-          qx_epi_deri<NA, NB, NC, ND, CMASK, XMASK, DINV, NCOMB, A0, A1, B0, B1, C0, C1, double>(A_, P1, P2, dv); \
+    if (active) {                                                                                 \
+      double dv[NCOMB * Shape::NEL];                                                              \
+      qx_core_##NAME(A_, P1, P2, u1, u2, dv);                                                     \
+      if (grad) {                                                                                 \
+        double gv[12];                                                                            \
+        qx_epi_dgrad<NA, NB, NC, ND, CMASK, XMASK, DINV, NCOMB, A0, A1, B0, B1, C0, C1>(A_, P1, P2, dv, gv); \
+        const int at[4] = {A_.pr.atom_a[P1], A_.pr.atom_b[P1], A_.pr.atom_a[P2], A_.pr.atom_b[P2]}; \
+        double* const gcp = qx_grad_copy(A_);                                                     \
+        for (int k = 0; k < 12; ++k) {                                                            \
+          const int slot = 3 * at[k / 3] + k % 3 - (lds ? 3 * lo : 0);                            \
+          if (lds) atomicAdd(&sg[slot], gv[k]);                                                   \
+          else qx_atomic_add(&gcp[slot], gv[k]);                                                  \
         }                                                                                         \
+      } else {                                                                                    \
+        qx_epi_deri<NA, NB, NC, ND, CMASK, XMASK, DINV, NCOMB, A0, A1, B0, B1, C0, C1, double>(A_, P1, P2, dv); \
       }                                                                                           \
     }                                                                                             \
     if (lds) {                                                                                    \
       __syncthreads();                                                                            \
-      qx_grad_flush(A_, sg, ng);                                                                  \
+      qx_grad_flush(A_, sg, lo, w);                                                               \
     }                                                                                             \
   }                                                                                               \
   extern "C" int qx_launcht_##NAME(const QxArgs* a, const QxLaunch* L) {                          \
@@ -770,23 +771,70 @@ __device__ __forceinline__ void qx_epi_djk(
 // groups of one part.
 #define QX_GRAD_NCOMB 9
 
+// Atom window of a block's LDS gradient accumulators: lo = the lowest atom
+// that an active lane of the block touches, w = 3 x the atoms from lo to the
+// highest one.  The block accumulates in LDS when the window fits in
+// QX_GRAD_LDS doubles (or the smaller grad_win).  In a fragment context (many molecules, one quartet
+// list) a block's quartets lie within one or two molecules, so the window
+// stays a few dozen atoms wide however many atoms the context holds.  Every
+// lane of the block calls it (two barriers); the results are block-uniform.
+__device__ __forceinline__ bool qx_grad_window(const QxArgs& A_, bool active, int P1, int P2, int& lo, int& w) {
+  __shared__ int win[2];
+  int mn = 0x7fffffff, mx = -1;
+  if (active) {
+    const int a0 = A_.pr.atom_a[P1], a1 = A_.pr.atom_b[P1], a2 = A_.pr.atom_a[P2], a3 = A_.pr.atom_b[P2];
+    mn = min(min(a0, a1), min(a2, a3));
+    mx = max(max(a0, a1), max(a2, a3));
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    mn = min(mn, __shfl_xor(mn, off));
+    mx = max(mx, __shfl_xor(mx, off));
+  }
+  if (threadIdx.x == 0) {
+    win[0] = 0x7fffffff;
+    win[1] = -1;
+  }
+  __syncthreads();
+  if (__lane_id() == 0 && mx >= 0) {
+    atomicMin(&win[0], mn);
+    atomicMax(&win[1], mx);
+  }
+  __syncthreads();
+  const int hi = win[1];
+  lo = hi >= 0 ? win[0] : 0;
+  w = hi >= 0 ? 3 * (hi - lo + 1) : 0;
+  const int cap = A_.e.grad_win > 0 && A_.e.grad_win < QX_GRAD_LDS ? A_.e.grad_win : QX_GRAD_LDS;
+  return hi >= 0 && w <= cap;
+}
+
+// The gradient copy a block adds to when it does not accumulate in LDS (its
+// own of the grad_copies copies, as in qx_grad_flush).
+__device__ __forceinline__ double* qx_grad_copy(const QxArgs& A_) {
+  const int nc = A_.e.grad_copies > 1 ? A_.e.grad_copies : 1;
+  return A_.e.grad + (size_t)(blockIdx.x % nc) * (size_t)(3 * A_.e.natom);
+}
+
 // Adds a lane's 12 centre x direction gradient terms gv (zero on inactive
-// lanes) to the block's LDS accumulators sg, or to the global ones when the
-// gradient does not fit in LDS.  With grad_wred (a per-class choice), a
+// lanes) to the block's LDS accumulators sg (window from atom lo), or to the
+// block's global gradient copy when the window does not fit in LDS.  With
+// grad_wred (a per-class choice), a
 // centre whose atom is the same on every active lane of the wavefront (the
 // bra centres when the wave shares a bra pair) is summed across the wave
 // first and added by one lane, so the lanes do not serialise on one LDS
 // address; that costs ~200 cross-lane operations per wave, which only the
 // cheapest classes repay.  Every lane calls it (converged).
-__device__ __forceinline__ void qx_grad_to_block(const QxArgs& A_, double* sg, bool lds, bool active,
+__device__ __forceinline__ void qx_grad_to_block(const QxArgs& A_, double* sg, bool lds, int lo, bool active,
                                                  const int (&at)[4], const double (&gv)[12]) {
+  double* const gcp = qx_grad_copy(A_);  // used when the block does not accumulate in LDS
+  const int off = lds ? 3 * lo : 0;      // slots are 3 * atom + direction - off
   if (!A_.e.grad_wred) {  // launch-uniform
     if (!active) return;
 #pragma unroll
     for (int k = 0; k < 12; ++k) {
-      const int slot = 3 * at[k / 3] + k % 3;
+      const int slot = 3 * at[k / 3] + k % 3 - off;
       if (lds) atomicAdd(&sg[slot], gv[k]);
-      else qx_atomic_add(&A_.e.grad[slot], gv[k]);
+      else qx_atomic_add(&gcp[slot], gv[k]);
     }
     return;
   }
@@ -796,16 +844,16 @@ __device__ __forceinline__ void qx_grad_to_block(const QxArgs& A_, double* sg, b
     const bool uni = qx_wave_uniform(at[c], active);  // wave-uniform value
 #pragma unroll
     for (int x = 0; x < 3; ++x) {
-      const int slot = 3 * at[c] + x;
+      const int slot = 3 * at[c] + x - off;
       if (uni) {
         const double v = qx_wave_sum(gv[3 * c + x]);
         if (lead && v != 0.0) {
           if (lds) atomicAdd(&sg[slot], v);
-          else qx_atomic_add(&A_.e.grad[slot], v);
+          else qx_atomic_add(&gcp[slot], v);
         }
       } else if (active && gv[3 * c + x] != 0.0) {
         if (lds) atomicAdd(&sg[slot], gv[3 * c + x]);
-        else qx_atomic_add(&A_.e.grad[slot], gv[3 * c + x]);
+        else qx_atomic_add(&gcp[slot], gv[3 * c + x]);
       }
     }
   }
@@ -815,28 +863,27 @@ __device__ __forceinline__ void qx_grad_to_block(const QxArgs& A_, double* sg, b
 // of the global ones: the blocks of a pass spread their atomics over the
 // copies instead of all meeting on the same 3N addresses (the runtime sums
 // the copies once per pass).
-__device__ __forceinline__ void qx_grad_flush(const QxArgs& A_, const double* sg, int ng) {
-  const int nc = A_.e.grad_copies > 1 ? A_.e.grad_copies : 1;
-  double* g = A_.e.grad + (size_t)(blockIdx.x % nc) * ng;
-  for (int i = threadIdx.x; i < ng; i += blockDim.x)
+__device__ __forceinline__ void qx_grad_flush(const QxArgs& A_, const double* sg, int lo, int w) {
+  double* g = qx_grad_copy(A_) + 3 * lo;
+  for (int i = threadIdx.x; i < w; i += blockDim.x)
     if (sg[i] != 0.0) qx_atomic_add(&g[i], sg[i]);
 }
 
 template <class Part>
 __device__ __forceinline__ void qx_grad_body(const QxArgs& A_, double* sg, Part part) {
-  const int ng = 3 * A_.e.natom;
-  const bool lds = ng <= QX_GRAD_LDS;
   const int bs = (int)blockDim.x;
   const long long t = (long long)blockIdx.x * bs + threadIdx.x;
   int u1 = 0, u2 = 0, P1 = -1, P2 = -1;
   bool active = t < A_.w.nquart;
   if (active) active = qx_locate(A_, t, u1, u2, P1, P2);
   if (qx_block_idle(A_, active)) return;
+  if (active && qx_single_atom(A_, P1, P2)) active = false;
+  int lo, w;
+  const bool lds = qx_grad_window(A_, active, P1, P2, lo, w);
   if (lds) {
-    for (int i = threadIdx.x; i < ng; i += bs) sg[i] = 0.0;
+    for (int i = threadIdx.x; i < w; i += bs) sg[i] = 0.0;
     __syncthreads();
   }
-  if (active && qx_single_atom(A_, P1, P2)) active = false;
   double gv[12];
 #pragma unroll
   for (int k = 0; k < 12; ++k) gv[k] = 0.0;
@@ -849,10 +896,10 @@ __device__ __forceinline__ void qx_grad_body(const QxArgs& A_, double* sg, Part 
     qx_grad_spread<7, 7, 1, QX_GRAD_NCOMB>(8.0 * qx_degeneracy(A_, P1, P2), g, gv);
     at[0] = A_.pr.atom_a[P1]; at[1] = A_.pr.atom_b[P1]; at[2] = A_.pr.atom_a[P2]; at[3] = A_.pr.atom_b[P2];
   }
-  qx_grad_to_block(A_, sg, lds, active, at, gv);
+  qx_grad_to_block(A_, sg, lds, lo, active, at, gv);
   if (lds) {
     __syncthreads();
-    qx_grad_flush(A_, sg, ng);
+    qx_grad_flush(A_, sg, lo, w);
   }
 }
 

@@ -127,6 +127,9 @@ struct qx_ctx {
   // per class: copies of the gradient accumulators its blocks spread their
   // flushes over (1..QX_GRAD_COPIES)
   int gcopy[QX_NUM_CLASSES_MAX] = {};
+  // widest atom window of the adjoint kernels' LDS gradient accumulators,
+  // in doubles (0: QX_GRAD_LDS)
+  int grad_win = 0;
   int* d_pair_frag = nullptr;            // fragment of each sorted pair (fragmented contexts)
   // fragmented contexts: first Cartesian function of each fragment id's range
   // and the widest range (the width of the J/K kernels' LDS K' rows)

@@ -1286,6 +1286,7 @@ int qx_eri_grad(qx_ctx* c, const double* D, double* grad, int precision, double 
     e.epi = QX_EPI_GRAD; e.nbf = (int)nc; e.natom = c->natom; e.coord = -1; e.D = Dcart;
     e.grad = begin_grad_copies(c, s);
     e.grad_copies = QX_GRAD_COPIES;
+    e.grad_win = c->grad_win;
     std::vector<Launch> v;
     if (precision == QX_PREC_MIXED) {
       add_mixed_launches(c, thr, 1, e, v);
@@ -1319,6 +1320,7 @@ int qx_eri_hess_dir(qx_ctx* c, const double* D, const double* v, double* out, vo
     e.epi = QX_EPI_GRAD; e.nbf = (int)nc; e.natom = c->natom; e.coord = -1; e.D = Dcart; e.tv = v;
     e.grad = begin_grad_copies(c, s);
     e.grad_copies = QX_GRAD_COPIES;
+    e.grad_win = c->grad_win;
     std::vector<Launch> L;
     add_dert_launches(c, e, L);
     run_launches(c, L, s, "eri_hess_dir");
@@ -1586,6 +1588,18 @@ int qx_set_grad_copies(qx_ctx* c, const int* copies) {
     for (int k = 0; k < QX_NUM_CLASSES; ++k)
       if (copies[k] < 1 || copies[k] > QX_GRAD_COPIES) return QX_EINVAL;
   for (int k = 0; k < QX_NUM_CLASSES; ++k) c->gcopy[k] = copies ? copies[k] : qx_gcopy_default[k];
+  return QX_OK;
+}
+
+int qx_set_grad_window(qx_ctx* c, int atoms) {
+  if (!c || atoms < 0 || 3 * atoms > QX_GRAD_LDS) return QX_EINVAL;
+  c->grad_win = 3 * atoms;
+  return QX_OK;
+}
+
+int qx_get_grad_window(const qx_ctx* c, int* atoms) {
+  if (!c || !atoms) return QX_EINVAL;
+  *atoms = (c->grad_win > 0 ? c->grad_win : QX_GRAD_LDS) / 3;
   return QX_OK;
 }
 

@@ -88,14 +88,26 @@ __device__ __forceinline__ bool mol_done(const double* sc, int b) { return sc[b 
 // layout of the 16x16x4 FP64 MFMA: lane l supplies A[l % 16][l / 16] and
 // B[l / 16][l % 16]; accumulator element r of lane l is D[4 r + l / 16][l % 16].
 // Molecules flagged done (sc) are skipped when skip is set.
+//
+// Split K: a single molecule's GEMM has few tiles (25 at n = 280 on a
+// 256-CU device), and each tile walks all of K in latency-bound steps.  With
+// nsplit > 1 blockIdx.z = molecule * nsplit + split, each split walks its
+// own range of K steps and stores its partial tile in part; the last split of
+// a tile to finish (a per-tile arrival counter in cnt) adds the partials in
+// split order -- the same sum whatever order they finished in -- applies
+// alpha and beta, writes C and puts the counter back to zero for the next
+// launch.  No workgroup waits for another.
 typedef double qx_sv4 __attribute__((ext_vector_type(4)));
 constexpr int BM = 64, BN = 64, BK = 32, GT = 256, PAD = 2;
+constexpr int KSPLIT_MAX = 8;
+constexpr int PART_TILE = BM * BN;  // doubles of one partial tile (16 per thread)
 
 template <bool TA, bool TB>
 __global__ __launch_bounds__(GT) void k_bgemm(int nm, const int* nb, const double* __restrict__ A,
                                               const double* __restrict__ B, double* __restrict__ C, double alpha,
-                                              double beta, const double* sc, int skip, int pslot) {
-  const int b = blockIdx.z;
+                                              double beta, const double* sc, int skip, int pslot, int nsplit,
+                                              double* part, unsigned* cnt) {
+  const int b = (int)blockIdx.z / nsplit, split = (int)blockIdx.z % nsplit;
   if (skip && mol_done(sc, b)) return;
   if (pslot >= 0 && sc[b * QS_N + QS_PCONV0 + pslot] != 0.0) return;  // purification already converged
   const int n = nb[b];
@@ -112,7 +124,9 @@ __global__ __launch_bounds__(GT) void k_bgemm(int nm, const int* nb, const doubl
   for (int x = 0; x < 2; ++x)
 #pragma unroll
     for (int y = 0; y < 2; ++y) acc[x][y] = qx_sv4{0.0, 0.0, 0.0, 0.0};
-  for (int k0 = 0; k0 < n; k0 += BK) {
+  const int nk = (n + BK - 1) / BK, per = (nk + nsplit - 1) / nsplit;  // K steps: all, per split
+  const int kbeg = split * per * BK, kend = min(n, (split + 1) * per * BK);
+  for (int k0 = kbeg; k0 < kend; k0 += BK) {
 #pragma unroll
     for (int r = 0; r < BK * BM / GT; ++r) {
       const int idx = tid + r * GT;
@@ -152,6 +166,29 @@ __global__ __launch_bounds__(GT) void k_bgemm(int nm, const int* nb, const doubl
           acc[ta][tb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ta], bb[tb], acc[ta][tb], 0, 0, 0);
     }
     __syncthreads();
+  }
+  if (nsplit > 1) {
+    // partial tile out (element q of every thread contiguous across the
+    // block), then the arrival count; the last split adds them all up
+    const size_t tile = ((size_t)b * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+    double* mine = part + (tile * nsplit + split) * PART_TILE;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) mine[q * GT + tid] = acc[q >> 3][(q >> 2) & 1][q & 3];
+    __threadfence();
+    __syncthreads();
+    __shared__ unsigned arrived;
+    if (tid == 0) arrived = atomicAdd(&cnt[tile], 1u);
+    __syncthreads();
+    if (arrived != (unsigned)(nsplit - 1)) return;  // block-uniform
+    __threadfence();
+    const double* all = part + tile * nsplit * PART_TILE;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      double v = 0.0;
+      for (int t = 0; t < nsplit; ++t) v += all[(size_t)t * PART_TILE + q * GT + tid];
+      acc[q >> 3][(q >> 2) & 1][q & 3] = v;
+    }
+    if (tid == 0) atomicExch(&cnt[tile], 0u);
   }
 #pragma unroll
   for (int ta = 0; ta < 2; ++ta)
@@ -480,6 +517,10 @@ struct qx_scf {
   // ntot x ntot
   double *Dfull = nullptr, *Jfull = nullptr, *Kfull = nullptr;
   double* sc = nullptr;
+  // split-K GEMMs (few tiles per launch): splits, partial tiles, arrival counters
+  int ksplit = 1;
+  double* gpart = nullptr;
+  unsigned* gcnt = nullptr;
   std::vector<void*> bufs;
   bool initialised = false;
   hipGraph_t graph = nullptr;
@@ -538,12 +579,15 @@ dim3 egrid(const qx_scf* s) { return dim3((unsigned)(((long long)s->nm * s->nm +
 // step pslot's GEMM (skipped for molecules whose purification converged)
 void gemm(qx_scf* s, bool ta, bool tb, const double* A, const double* B, double* C, double alpha, double beta,
           hipStream_t st, int pslot = -1) {
-  const dim3 grid((unsigned)((s->nm + BN - 1) / BN), (unsigned)((s->nm + BM - 1) / BM), (unsigned)s->nmol);
+  const int ns = s->ksplit;
+  const dim3 grid((unsigned)((s->nm + BN - 1) / BN), (unsigned)((s->nm + BM - 1) / BM), (unsigned)(s->nmol * ns));
   const int ps = pslot < 0 ? -1 : (pslot & 1);
-  if (!ta && !tb) hipLaunchKernelGGL((k_bgemm<false, false>), grid, dim3(GT), 0, st, s->nm, s->nb, A, B, C, alpha, beta, s->sc, 1, ps);
-  else if (ta && !tb) hipLaunchKernelGGL((k_bgemm<true, false>), grid, dim3(GT), 0, st, s->nm, s->nb, A, B, C, alpha, beta, s->sc, 1, ps);
-  else if (!ta && tb) hipLaunchKernelGGL((k_bgemm<false, true>), grid, dim3(GT), 0, st, s->nm, s->nb, A, B, C, alpha, beta, s->sc, 1, ps);
-  else hipLaunchKernelGGL((k_bgemm<true, true>), grid, dim3(GT), 0, st, s->nm, s->nb, A, B, C, alpha, beta, s->sc, 1, ps);
+  double* pt = s->gpart;
+  unsigned* ct = s->gcnt;
+  if (!ta && !tb) hipLaunchKernelGGL((k_bgemm<false, false>), grid, dim3(GT), 0, st, s->nm, s->nb, A, B, C, alpha, beta, s->sc, 1, ps, ns, pt, ct);
+  else if (ta && !tb) hipLaunchKernelGGL((k_bgemm<true, false>), grid, dim3(GT), 0, st, s->nm, s->nb, A, B, C, alpha, beta, s->sc, 1, ps, ns, pt, ct);
+  else if (!ta && tb) hipLaunchKernelGGL((k_bgemm<false, true>), grid, dim3(GT), 0, st, s->nm, s->nb, A, B, C, alpha, beta, s->sc, 1, ps, ns, pt, ct);
+  else hipLaunchKernelGGL((k_bgemm<true, true>), grid, dim3(GT), 0, st, s->nm, s->nb, A, B, C, alpha, beta, s->sc, 1, ps, ns, pt, ct);
   check(hipGetLastError(), "scf gemm");
 }
 
@@ -684,6 +728,22 @@ int qx_scf_create(qx_ctx* ctx, int nmol, int nm, const int* nbf, const int* offs
     // DIIS ring's unfilled slots enter its combination as zeros)
     for (double* p : {s->D, s->Dold, s->Dnew, s->F, s->Fe, s->J, s->K, s->T1, s->T2, s->err, s->X, s->Xb, s->X2})
       QS_HIP(hipMemset(p, 0, nb2 * sizeof(double)));
+    // split K when one launch has too few tiles to spread over the CUs
+    // (about 256 workgroups wanted): at most KSPLIT_MAX splits, each with at
+    // least one K step; the arrival counters start (and are left) at zero
+    {
+      const int tiles = (nm + BM - 1) / BM, nk = (nm + BK - 1) / BK;
+      const long long wg = (long long)tiles * tiles * nmol;
+      int ns = (int)std::min<long long>(KSPLIT_MAX, std::max<long long>(1, (256 + wg - 1) / wg));
+      ns = std::min(ns, nk);
+      const int per = (nk + ns - 1) / ns;
+      s->ksplit = (nk + per - 1) / per;  // no empty split
+      if (s->ksplit > 1) {
+        s->gpart = salloc<double>(s, (size_t)wg * s->ksplit * PART_TILE);
+        s->gcnt = salloc<unsigned>(s, (size_t)wg);
+        QS_HIP(hipMemset(s->gcnt, 0, (size_t)wg * sizeof(unsigned)));
+      }
+    }
     QS_HIP(hipMemset(s->Fh, 0, nb2 * s->m * sizeof(double)));
     QS_HIP(hipMemset(s->Eh, 0, nb2 * s->m * sizeof(double)));
     QS_HIP(hipMemset(s->Bm, 0, (size_t)nmol * s->m * s->m * sizeof(double)));

@@ -192,6 +192,16 @@ class EriEngine:
             raise ValueError("one copy count per kernel class")
         check(self.L.qx_set_grad_copies(self.ctx, f.ctypes.data_as(ctypes.c_void_p)), self.ctx, "set_grad_copies")
 
+    def grad_window(self):
+        """Widest atom window of the adjoint kernels' LDS gradient accumulators."""
+        a = ctypes.c_int(0)
+        check(self.L.qx_get_grad_window(self.ctx, ctypes.byref(a)), self.ctx)
+        return a.value
+
+    def set_grad_window(self, atoms: int = 0):
+        """Sets it (0..256 atoms; 0: the default, 256)."""
+        check(self.L.qx_set_grad_window(self.ctx, int(atoms)), self.ctx, "set_grad_window")
+
     def set_fp32_classes(self, pass_: int, flags=None):
         """Sets the per-class FP32 policy of a mixed pass (None: built-in default)."""
         if flags is None:

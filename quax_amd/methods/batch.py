@@ -36,8 +36,10 @@ def batched_rhf(molecules, basis_name="even-tempered", pure=True, ints_tol=1e-12
     device, converged molecules skipped; a molecule whose DIIS error stalls at
     the purification's rounding floor finishes in the host-driven loop);
     otherwise the host-driven lock-step loop (_eager_scf)."""
+    import time
     import torch
     from ..engine import EriEngine
+    t_start = time.perf_counter()
     dev = torch.device("cuda", device)
     dt = torch.float64
     prec = QX_PREC_MIXED if precision == "mixed" else QX_PREC_FP64
@@ -70,13 +72,14 @@ def batched_rhf(molecules, basis_name="even-tempered", pure=True, ints_tol=1e-12
         Z = [m.charge(i) for i in range(m.natom())]
         Enuc.append(nuclear_repulsion(m.geometry(), Z))
     blk = [slice(int(off[k]), int(off[k + 1])) for k in range(B)]
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
+    timing = {"setup_s": t0 - t_start}
     if graph:
-        import time
-        t0 = time.perf_counter()
         E, D, C_all, eps_all, iters, done, stalled = _graph_scf(eng, H, S, A, ndocc, Enuc, off, blk, ntot, conv,
                                                                 maxit, diis_space, prec, fp32_threshold,
                                                                 purify_iters, dev)
-        timing = {"graph_s": time.perf_counter() - t0, "stalled": int(stalled.sum())}
+        timing.update(graph_s=time.perf_counter() - t0, stalled=int(stalled.sum()))
         if stalled.any():
             # the last iterations of molecules whose purified densities hit
             # their rounding floor: eigh-based, from the graph's densities
@@ -93,8 +96,9 @@ def batched_rhf(molecules, basis_name="even-tempered", pure=True, ints_tol=1e-12
                        fp32_threshold, timing)
     E, D, C_all, eps_all, iters, done = _eager_scf(eng, H, S, A, ndocc, Enuc, blk, nb, ntot, conv, maxit, diis_space,
                                                    prec, fp32_threshold, dev)
+    timing["scf_s"] = time.perf_counter() - t0
     return _finish(eng, one, molecules, D, C_all, eps_all, blk, aof, ndocc, E, iters, done, nb, gradient, prec,
-                   fp32_threshold)
+                   fp32_threshold, timing)
 
 
 def _eager_scf(eng, H, S, A, ndocc, Enuc, blk, nb, ntot, conv, maxit, diis_space, prec, fp32_threshold, dev,
@@ -213,9 +217,15 @@ def _graph_scf(eng, H, S, A, ndocc, Enuc, off, blk, ntot, conv, maxit, diis_spac
 
 def _finish(eng, one, molecules, D, C_all, eps_all, blk, aof, ndocc, E, iters, done, nb, gradient, prec,
             fp32_threshold, timing=None):
+    import time
+    import torch
+    t0 = time.perf_counter()
     grads = None
     if gradient:
         g2 = eng.eri_grad(D, precision=prec, fp32_threshold=fp32_threshold).reshape(-1, 3)
+        if timing is not None:
+            torch.cuda.synchronize()
+            timing["eri_grad_s"] = time.perf_counter() - t0
         grads = []
         for k, (m, e1) in enumerate(zip(molecules, one)):
             b_, nd = blk[k], ndocc[k]
@@ -230,6 +240,8 @@ def _finish(eng, one, molecules, D, C_all, eps_all, blk, aof, ndocc, E, iters, d
         e1.close()
     eng.close()
     aux = {"iterations": iters.tolist(), "converged": done.tolist(), "nbf": nb}
-    if timing:
+    if timing is not None:
+        torch.cuda.synchronize()
         aux.update(timing)
+        aux["finish_s"] = time.perf_counter() - t0
     return list(E), grads, aux

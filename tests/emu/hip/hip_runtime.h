@@ -119,6 +119,20 @@ inline double unsafeAtomicAdd(double* p, double v) {
 }
 inline double atomicAdd(double* p, double v) { return std::atomic_ref<double>(*p).fetch_add(v); }
 inline float atomicAdd(float* p, float v) { return std::atomic_ref<float>(*p).fetch_add(v); }
+inline int atomicMin(int* p, int v) {
+  std::atomic_ref<int> a(*p);
+  int o = a.load();
+  while (v < o && !a.compare_exchange_weak(o, v)) {
+  }
+  return o;
+}
+inline int atomicMax(int* p, int v) {
+  std::atomic_ref<int> a(*p);
+  int o = a.load();
+  while (v > o && !a.compare_exchange_weak(o, v)) {
+  }
+  return o;
+}
 
 namespace emu {
 // dynamic shared memory of the running grid: exactly the launch's byte

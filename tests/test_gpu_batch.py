@@ -210,3 +210,75 @@ def test_batched_graph_scf_seeded_molecules(gpu):
         assert abs(E[k] - Ek) < 1e-9
         assert np.abs(G[k] - gk).max() < 1e-7
     assert compared >= 6
+
+
+def test_gradient_window_paths_agree(gpu):
+    """The adjoint kernels accumulate a block's gradient terms in LDS over
+    the atom window its quartets touch, or in global memory when the window
+    is wider than the limit: a limit of 1 or 2 atoms sends all or some
+    blocks down the global path, and the gradient (and the forward-over-
+    reverse directional derivative, through the tangent kernels) must not
+    change beyond summation order."""
+    from test_gpu_eri import small_d_cluster
+    from quax_amd.engine import EriEngine
+    mol, basis = small_d_cluster()
+    rng = np.random.default_rng(11)
+    n = basis.nbf()
+    X = rng.standard_normal((n, n)) * 0.3
+    D = X @ X.T / n
+    v = rng.standard_normal(3 * mol.natom())
+    with EriEngine(mol, basis, ints_tol=1e-14) as eng:
+        assert eng.grad_window() == 256
+        g0, h0 = eng.eri_grad(D).cpu().numpy(), eng.eri_hess_dir(D, v).cpu().numpy()
+        for atoms in (1, 2):
+            eng.set_grad_window(atoms)
+            assert eng.grad_window() == atoms
+            g, h = eng.eri_grad(D).cpu().numpy(), eng.eri_hess_dir(D, v).cpu().numpy()
+            print(f"  window {atoms}: max dg {np.abs(g - g0).max():.1e}, max dh {np.abs(h - h0).max():.1e}",
+                  flush=True)
+            assert np.abs(g - g0).max() <= 1e-12 * max(1.0, np.abs(g0).max())
+            assert np.abs(h - h0).max() <= 1e-12 * max(1.0, np.abs(h0).max())
+        eng.set_grad_window(0)
+        assert eng.grad_window() == 256
+        with pytest.raises(RuntimeError):
+            eng.set_grad_window(257)
+
+
+def test_fragment_gradient_beyond_256_atoms(gpu):
+    """96 molecules (288 atoms, more than the 256 an LDS window holds) as
+    fragments of one context: every block's window stays within one or two
+    molecules, and each molecule's gradient equals its own context's."""
+    from quax_amd.engine import EriEngine
+    protos = [Molecule(["O", "H", "H"], [[0.0, 0.0, 0.0], [0.0, 1.43, 1.1], [0.0, -1.43, 1.1]]),
+              Molecule(["O", "H", "H"], [[0.1, 0.0, 0.0], [0.0, 1.52, 0.9], [0.2, -1.31, 1.2]]),
+              Molecule(["C", "O", "H"], [[0.0, 0.0, 0.0], [0.0, 0.0, 2.2], [0.0, 1.9, -0.6]]),
+              Molecule(["H", "O", "H"], [[1.2, 0.3, 0.0], [0.0, 0.0, 0.0], [-0.9, 1.0, 0.3]])]
+    rng = np.random.default_rng(5)
+    pb = [BasisSet.build(m, "sto-3g") for m in protos]
+    pD = []
+    for b in pb:
+        X = rng.standard_normal((b.nbf(), b.nbf())) * 0.3
+        pD.append(X @ X.T / b.nbf())
+    mols, which = [], []
+    for k in range(96):
+        p = protos[k % 4]
+        shift = np.array([20.0 * (k % 6), 20.0 * ((k // 6) % 4), 20.0 * (k // 24)])
+        mols.append(Molecule([p.symbol(i) for i in range(3)], np.asarray(p.geometry()).reshape(-1, 3) + shift))
+        which.append(k % 4)
+    bases = [BasisSet.build(m, "sto-3g") for m in mols]
+    comb, cb, frag = _combined(mols, bases)
+    assert comb.natom() == 288
+    nb = [b.nbf() for b in bases]
+    off = np.concatenate([[0], np.cumsum(nb)]).astype(int)
+    D = np.zeros((off[-1], off[-1]))
+    for k, w in enumerate(which):
+        D[off[k]:off[k + 1], off[k]:off[k + 1]] = pD[w]
+    with EriEngine(comb, cb, ints_tol=1e-12, fragments=frag) as eng:
+        g = eng.eri_grad(D).cpu().numpy()
+    ref = []
+    for m, b, Dk in zip(protos, pb, pD):
+        with EriEngine(m, b, ints_tol=1e-12) as e1:
+            ref.append(e1.eri_grad(Dk).cpu().numpy())
+    err = max(np.abs(g[9 * k:9 * k + 9] - ref[w]).max() for k, w in enumerate(which))
+    print(f"  288 atoms in 96 fragments: max gradient difference {err:.1e}", flush=True)
+    assert err < 1e-10
