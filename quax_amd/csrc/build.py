@@ -13,9 +13,11 @@ import argparse
 import concurrent.futures as cf
 import hashlib
 import os
+import re
 import shutil
 import subprocess
 import sys
+import tempfile
 import time
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -27,6 +29,7 @@ LIB = os.path.join(LIBDIR, "libquax_hip.so")
 ROOT = os.path.dirname(PKG)
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 ARCH = "gfx950"
 FLAGS = ["-std=c++17", "-O3", f"--offload-arch={ARCH}", "-fPIC", "-munsafe-fp-atomics",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -112,6 +115,52 @@ def build_h5(verbose=True):
     return H5_LIB
 
 
+# Scratch per lane allowed to the streaming J/K and adjoint-gradient kernels,
+# which run on up to four streams: a side stream's hardware queue that has to
+# grow its scratch past ~8 KB per lane, once the process holds scratch on its
+# other queues, has been refused by the HSA runtime (the queue aborts with
+# HSA_STATUS_ERROR_OUT_OF_RESOURCES).  Larger kernels run on the caller's
+# stream only.
+STREAMING_SCRATCH_MAX = 6144
+LLVM_BIN = os.path.join(ROCM, "llvm", "bin")
+
+
+def kernel_scratch(obj: str, tmp: str) -> list:
+    """(kernel name, scratch bytes per lane) of the gfx950 code object in a
+    compiled object file (its .hip_fatbin offload bundle)."""
+    fb, co = os.path.join(tmp, "fb.bin"), os.path.join(tmp, "k.co")
+    subprocess.run([os.path.join(LLVM_BIN, "llvm-objcopy"), f"--dump-section=.hip_fatbin={fb}", obj, os.devnull],
+                   check=True, capture_output=True)
+    subprocess.run([os.path.join(LLVM_BIN, "clang-offload-bundler"), "--unbundle", "--type=o", f"--input={fb}",
+                    f"--targets=hipv4-amdgcn-amd-amdhsa--{ARCH}", f"--output={co}"], check=True, capture_output=True)
+    notes = subprocess.run([os.path.join(LLVM_BIN, "llvm-readelf"), "--notes", co], check=True,
+                           capture_output=True, text=True).stdout
+    names = re.findall(r"\.name:\s+(\S+)", notes)
+    sizes = [int(x) for x in re.findall(r"\.private_segment_fixed_size:\s+(\d+)", notes)]
+    return list(zip(names, sizes))
+
+
+def check_streaming_scratch(objs, verbose=True):
+    """Fails the build when a streaming J/K (cls_*_j*) or adjoint-gradient
+    (cls_*_g*) kernel needs more than STREAMING_SCRATCH_MAX bytes of scratch
+    per lane."""
+    if not os.path.exists(os.path.join(LLVM_BIN, "clang-offload-bundler")):
+        print("scratch check skipped: no clang-offload-bundler", flush=True)
+        return
+    streaming = [o for o in objs if re.match(r"cls_\d{4}_[jg]\d+\.hip\.", os.path.basename(o))]
+    worst = ("", 0)
+    with tempfile.TemporaryDirectory() as tmp:
+        for o in streaming:
+            for name, size in kernel_scratch(o, tmp):
+                if size > worst[1]:
+                    worst = (name, size)
+    if worst[1] > STREAMING_SCRATCH_MAX:
+        raise RuntimeError(f"streaming kernel {worst[0]} needs {worst[1]} B of scratch per lane "
+                           f"(at most {STREAMING_SCRATCH_MAX} allowed on side streams)")
+    if verbose:
+        print(f"streaming kernels: at most {worst[1]} B of scratch per lane ({worst[0]})", flush=True)
+
+
 def build(force=False, jobs=None, lmax=2, fp32=True, verbose=True, budget=None, out=None, waves=None,
           csplit=None):
     """budget / waves: one derivative node budget / waves-per-SIMD target for
@@ -150,6 +199,7 @@ def build(force=False, jobs=None, lmax=2, fp32=True, verbose=True, budget=None, 
                 print(f"  compiled {os.path.basename(futs[f])} in {dt:.1f} s", flush=True)
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", lib + ".tmp"] + sorted(objs)
     subprocess.run(cmd, check=True)
+    check_streaming_scratch(objs, verbose)
     os.replace(lib + ".tmp", lib)
     if verbose:
         print(f"built {lib} ({len(objs)} objects, {time.time() - t0:.1f} s)", flush=True)

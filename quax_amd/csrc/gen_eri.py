@@ -312,8 +312,11 @@ JK_LAZY_D_MIN = int(os.environ.get("QX_JK_LAZY_D_MIN", "324"))
 # split over ranges of a components (as the derivative parts are).  Measured
 # on cluster10: splitting (dd|dp) into 3 and (dd|dd) into 6 parts made their
 # J/K kernels 1.2x and 1.8x slower (recomputed recurrences cost more than
-# the 5-11 KB of spills saved), so no current class is split.
-JK_NODES = 8000
+# the 5-11 KB of spills saved).  The unsplit (dd|dd) body (7614 nodes) needs
+# 9.2 KB of scratch per lane, and the HSA runtime has refused to provide
+# scratch above ~8 KB per lane for some dispatches (the queue aborts with
+# HSA_STATUS_ERROR_OUT_OF_RESOURCES), so (dd|dd) alone is split in two.
+JK_NODES = 7000
 
 
 def jk_parts(l, budget=None):

@@ -715,44 +715,47 @@ __device__ __forceinline__ void qx_epi_djk(
 // holds sum_x tv_x d^2(ab|cd)/dR_x dR_y per centre x direction slot y.
 // Epilogues: DFULL (second-derivative tensor slices through slot_map) or
 // GRAD (the adjoint contraction with the density pair products, per block in
-// LDS).  The GRAD path's barriers run in block-uniform control flow.
+// LDS over all 3N slots; a context of more than 256 atoms adds to global
+// memory instead -- the adjoint kernels' atom windows are left out here, as
+// they would lift the largest tangent kernels above 8 KB of scratch per
+// lane).  The GRAD path's barriers run in block-uniform control flow.
 #define QX_DEFINE_DERT_KERNEL(NAME, N, NA, NB, NC, ND, CMASK, XMASK, DINV, NCOMB, A0, A1, B0, B1, C0, C1) \
   __global__ __launch_bounds__(QX_BLOCK) void qx_kt_##NAME(QxArgs A_) {                           \
     __shared__ double sg[QX_GRAD_LDS];                                                            \
     using Shape = QxPartShape<NA, NB, NC, ND, A0, A1, B0, B1, C0, C1>;                            \
     static_assert(NCOMB * Shape::NEL * 8 <= 16384,                                                \
                   "per-lane derivative block must stay small (scratch is allocated per lane)");    \
-    const bool grad = A_.e.epi == QX_EPI_GRAD; /* launch-uniform */                               \
-    const long long t = (long long)blockIdx.x * QX_BLOCK + threadIdx.x;                          \
-    int u1 = 0, u2 = 0, P1 = -1, P2 = -1;                                                         \
-    bool active = t < A_.w.nquart;                                                                \
-    if (active) active = qx_locate(A_, t, u1, u2, P1, P2) && !qx_single_atom(A_, P1, P2);         \
-    int lo = 0, w = 0;                                                                            \
-    const bool lds = grad && qx_grad_window(A_, active, P1, P2, lo, w);                           \
+    const bool grad = A_.e.epi == QX_EPI_GRAD;                                                    \
+    const int ng = 3 * A_.e.natom;                                                                \
+    const bool lds = grad && ng <= QX_GRAD_LDS;                                                   \
     if (lds) {                                                                                    \
-      for (int i = threadIdx.x; i < w; i += QX_BLOCK) sg[i] = 0.0;                                \
+      for (int i = threadIdx.x; i < ng; i += QX_BLOCK) sg[i] = 0.0;                               \
       __syncthreads();                                                                            \
     }                                                                                             \
-    if (active) {                                                                                 \
-      double dv[NCOMB * Shape::NEL];                                                              \
-      qx_core_##NAME(A_, P1, P2, u1, u2, dv);                                                     \
-      if (grad) {                                                                                 \
-        double gv[12];                                                                            \
-        qx_epi_dgrad<NA, NB, NC, ND, CMASK, XMASK, DINV, NCOMB, A0, A1, B0, B1, C0, C1>(A_, P1, P2, dv, gv); \
-        const int at[4] = {A_.pr.atom_a[P1], A_.pr.atom_b[P1], A_.pr.atom_a[P2], A_.pr.atom_b[P2]}; \
-        double* const gcp = qx_grad_copy(A_);                                                     \
-        for (int k = 0; k < 12; ++k) {                                                            \
-          const int slot = 3 * at[k / 3] + k % 3 - (lds ? 3 * lo : 0);                            \
-          if (lds) atomicAdd(&sg[slot], gv[k]);                                                   \
-          else qx_atomic_add(&gcp[slot], gv[k]);                                                  \
+    const long long t = (long long)blockIdx.x * QX_BLOCK + threadIdx.x;                          \
+    if (t < A_.w.nquart) {                                                                        \
+      int u1, u2, P1, P2;                                                                         \
+      qx_locate(A_, t, u1, u2, P1, P2);                                                           \
+      if (!qx_single_atom(A_, P1, P2)) {                                                          \
+        double dv[NCOMB * Shape::NEL];                                                            \
+        qx_core_##NAME(A_, P1, P2, u1, u2, dv);                                                   \
+        if (grad) {                                                                               \
+          double gv[12];                                                                          \
+          qx_epi_dgrad<NA, NB, NC, ND, CMASK, XMASK, DINV, NCOMB, A0, A1, B0, B1, C0, C1>(A_, P1, P2, dv, gv); \
+          const int at[4] = {A_.pr.atom_a[P1], A_.pr.atom_b[P1], A_.pr.atom_a[P2], A_.pr.atom_b[P2]}; \
+          for (int k = 0; k < 12; ++k) {                                                          \
+            const int slot = 3 * at[k / 3] + k % 3;                                               \
+            if (lds) atomicAdd(&sg[slot], gv[k]);                                                 \
+            else qx_atomic_add(&A_.e.grad[slot], gv[k]);                                          \
+          }                                                                                       \
+        } else {                                                                                  \
+          qx_epi_deri<NA, NB, NC, ND, CMASK, XMASK, DINV, NCOMB, A0, A1, B0, B1, C0, C1, double>(A_, P1, P2, dv); \
         }                                                                                         \
-      } else {                                                                                    \
-        qx_epi_deri<NA, NB, NC, ND, CMASK, XMASK, DINV, NCOMB, A0, A1, B0, B1, C0, C1, double>(A_, P1, P2, dv); \
       }                                                                                           \
     }                                                                                             \
     if (lds) {                                                                                    \
       __syncthreads();                                                                            \
-      qx_grad_flush(A_, sg, lo, w);                                                               \
+      qx_grad_flush(A_, sg, 0, ng);                                                               \
     }                                                                                             \
   }                                                                                               \
   extern "C" int qx_launcht_##NAME(const QxArgs* a, const QxLaunch* L) {                          \
@@ -772,19 +775,27 @@ __device__ __forceinline__ void qx_epi_djk(
 #define QX_GRAD_NCOMB 9
 
 // Atom window of a block's LDS gradient accumulators: lo = the lowest atom
-// that an active lane of the block touches, w = 3 x the atoms from lo to the
-// highest one.  The block accumulates in LDS when the window fits in
-// QX_GRAD_LDS doubles (or the smaller grad_win).  In a fragment context (many molecules, one quartet
-// list) a block's quartets lie within one or two molecules, so the window
-// stays a few dozen atoms wide however many atoms the context holds.  Every
-// lane of the block calls it (two barriers); the results are block-uniform.
-__device__ __forceinline__ bool qx_grad_window(const QxArgs& A_, bool active, int P1, int P2, int& lo, int& w) {
+// that an active lane of the block touches (its centres' atoms at), w = 3 x
+// the atoms from lo to the highest one; the block accumulates in LDS when the
+// window fits in QX_GRAD_LDS doubles (or the smaller grad_win).  A context
+// whose atoms all fit takes the whole gradient as its window, with no
+// reduction.  In a fragment context (many molecules, one quartet list) a
+// block's quartets lie within one or two molecules, so the window stays a few
+// dozen atoms wide however many atoms the context holds.  Called after the
+// quartets are evaluated (nothing of it is live across them) by every lane of
+// the block; the reduction has two barriers and block-uniform results.
+__device__ __forceinline__ bool qx_grad_window(const QxArgs& A_, bool active, const int (&at)[4], int& lo, int& w) {
+  const int cap = A_.e.grad_win > 0 && A_.e.grad_win < QX_GRAD_LDS ? A_.e.grad_win : QX_GRAD_LDS;
+  if (3 * A_.e.natom <= cap) {  // launch-uniform: every atom fits, no reduction needed
+    lo = 0;
+    w = 3 * A_.e.natom;
+    return true;
+  }
   __shared__ int win[2];
   int mn = 0x7fffffff, mx = -1;
   if (active) {
-    const int a0 = A_.pr.atom_a[P1], a1 = A_.pr.atom_b[P1], a2 = A_.pr.atom_a[P2], a3 = A_.pr.atom_b[P2];
-    mn = min(min(a0, a1), min(a2, a3));
-    mx = max(max(a0, a1), max(a2, a3));
+    mn = min(min(at[0], at[1]), min(at[2], at[3]));
+    mx = max(max(at[0], at[1]), max(at[2], at[3]));
   }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
@@ -804,7 +815,6 @@ __device__ __forceinline__ bool qx_grad_window(const QxArgs& A_, bool active, in
   const int hi = win[1];
   lo = hi >= 0 ? win[0] : 0;
   w = hi >= 0 ? 3 * (hi - lo + 1) : 0;
-  const int cap = A_.e.grad_win > 0 && A_.e.grad_win < QX_GRAD_LDS ? A_.e.grad_win : QX_GRAD_LDS;
   return hi >= 0 && w <= cap;
 }
 
@@ -878,12 +888,6 @@ __device__ __forceinline__ void qx_grad_body(const QxArgs& A_, double* sg, Part 
   if (active) active = qx_locate(A_, t, u1, u2, P1, P2);
   if (qx_block_idle(A_, active)) return;
   if (active && qx_single_atom(A_, P1, P2)) active = false;
-  int lo, w;
-  const bool lds = qx_grad_window(A_, active, P1, P2, lo, w);
-  if (lds) {
-    for (int i = threadIdx.x; i < w; i += bs) sg[i] = 0.0;
-    __syncthreads();
-  }
   double gv[12];
 #pragma unroll
   for (int k = 0; k < 12; ++k) gv[k] = 0.0;
@@ -895,6 +899,12 @@ __device__ __forceinline__ void qx_grad_body(const QxArgs& A_, double* sg, Part 
     part(P1, P2, u1, u2, g);
     qx_grad_spread<7, 7, 1, QX_GRAD_NCOMB>(8.0 * qx_degeneracy(A_, P1, P2), g, gv);
     at[0] = A_.pr.atom_a[P1]; at[1] = A_.pr.atom_b[P1]; at[2] = A_.pr.atom_a[P2]; at[3] = A_.pr.atom_b[P2];
+  }
+  int lo, w;
+  const bool lds = qx_grad_window(A_, active, at, lo, w);
+  if (lds) {
+    for (int i = threadIdx.x; i < w; i += bs) sg[i] = 0.0;
+    __syncthreads();
   }
   qx_grad_to_block(A_, sg, lds, lo, active, at, gv);
   if (lds) {

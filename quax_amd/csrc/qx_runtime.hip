@@ -246,8 +246,18 @@ void check_launch(int rc, const char* what);
 // all of them at the end, so the pass is ordered on `s` like one kernel.
 // With profiling on, each kernel's HIP events sit on the stream it runs on:
 // concurrent kernels are timed in place (one stream times each alone).
-void run_launches(qx_ctx* c, std::vector<Launch>& v, hipStream_t s, const char* what) {
-  const int ns = std::max(1, std::min<int>(c->nstreams, (int)v.size()));
+//
+// side = false keeps the pass on the caller's stream.  The passes whose
+// kernels need more than ~6 KB of scratch per lane (derivative and
+// second-derivative tensors, the tangent kernels: up to 8.3 KB) run there:
+// each hardware queue grows its own scratch to the largest per-lane size it
+// has run, and growing a side stream's queue past ~8 KB per lane after the
+// process already holds scratch on its other queues has been refused by the
+// HSA runtime (HSA_STATUS_ERROR_OUT_OF_RESOURCES).  The caller's queue has
+// already run the largest kernels, the array-form ones of context set-up.
+// The streaming J/K and adjoint-gradient kernels stay under 6 KB per lane.
+void run_launches(qx_ctx* c, std::vector<Launch>& v, hipStream_t s, const char* what, bool side) {
+  const int ns = side ? std::max(1, std::min<int>(c->nstreams, (int)v.size())) : 1;
   if (ns == 1) {
     for (auto& x : v) check_launch(x.go(s), what);
     return;
@@ -1027,7 +1037,7 @@ static void add_mixed_launches(qx_ctx* c, double thr, int pass, const QxEpiDev& 
 static void launch_deriv_all(qx_ctx* c, const QxEpiDev& e, int fp32, hipStream_t s) {
   std::vector<Launch> v;
   add_deriv_launches(c, c->work, e, fp32, v);
-  run_launches(c, v, s, "eri derivative");
+  run_launches(c, v, s, "eri derivative", false);
 }
 
 static void deriv_tensor(qx_ctx* c, int coord, double* out, hipStream_t s) {
@@ -1104,7 +1114,7 @@ static void deriv2_tensor(qx_ctx* c, long long want, double* out, hipStream_t s)
     e.slot_map = static_cast<const int*>(dmap.p) + (size_t)x * m;
     std::vector<Launch> v;
     add_dert_launches(c, e, v);
-    run_launches(c, v, s, "eri second derivative");
+    run_launches(c, v, s, "eri second derivative", false);
   }
   if (c->pure) ao_tensor4_from_cart(c, dst, out, nslices, s);
   QX_HIP(hipStreamSynchronize(s));  // dtv / dmap are freed on return
@@ -1198,7 +1208,7 @@ int qx_jk_build_masked(qx_ctx* c, const double* D, double* J, double* K, int pre
     } else {
       add_jk_launches(c, c->work, e, 0, v);
     }
-    run_launches(c, v, s, "jk_build");
+    run_launches(c, v, s, "jk_build", true);
     if (c->pure) {
       check_launch(qxk_sym_half(Jp, Js, (int)nc, s), "sym");
       check_launch(qxk_sym_half(Kp, Ks, (int)nc, s), "sym");
@@ -1293,7 +1303,7 @@ int qx_eri_grad(qx_ctx* c, const double* D, double* grad, int precision, double 
     } else {
       add_deriv_launches(c, c->work, e, 0, v);
     }
-    run_launches(c, v, s, "eri_grad");
+    run_launches(c, v, s, "eri_grad", true);
     check_launch(qxk_add_copies(c->gcopies, QX_GRAD_COPIES, 3 * c->natom, grad, s), "gradient copies");
     end_ws(c, s);
     return QX_OK;
@@ -1323,7 +1333,7 @@ int qx_eri_hess_dir(qx_ctx* c, const double* D, const double* v, double* out, vo
     e.grad_win = c->grad_win;
     std::vector<Launch> L;
     add_dert_launches(c, e, L);
-    run_launches(c, L, s, "eri_hess_dir");
+    run_launches(c, L, s, "eri_hess_dir", false);
     check_launch(qxk_add_copies(c->gcopies, QX_GRAD_COPIES, 3 * c->natom, out, s), "gradient copies");
     end_ws(c, s);
     return QX_OK;
@@ -1353,7 +1363,7 @@ int qx_jk_deriv_dir(qx_ctx* c, const double* D, const double* v, double* J, doub
     e.tv = v;
     std::vector<Launch> L;
     add_deriv_launches(c, c->work, e, 0, L);
-    run_launches(c, L, s, "jk_deriv_dir");
+    run_launches(c, L, s, "jk_deriv_dir", false);
     if (c->pure) {
       check_launch(qxk_sym_half(Jp, Js, (int)nc, s), "sym");
       check_launch(qxk_sym_half(Kp, Ks, (int)nc, s), "sym");

@@ -90,26 +90,31 @@ __device__ __forceinline__ bool mol_done(const double* sc, int b) { return sc[b 
 // Molecules flagged done (sc) are skipped when skip is set.
 //
 // Split K: a single molecule's GEMM has few tiles (25 at n = 280 on a
-// 256-CU device), and each tile walks all of K in latency-bound steps.  With
-// nsplit > 1 blockIdx.z = molecule * nsplit + split, each split walks its
-// own range of K steps and stores its partial tile in part; the last split of
-// a tile to finish (a per-tile arrival counter in cnt) adds the partials in
-// split order -- the same sum whatever order they finished in -- applies
-// alpha and beta, writes C and puts the counter back to zero for the next
-// launch.  No workgroup waits for another.
+// 256-CU device), and each tile's four wavefronts walk all of K: the FP64
+// MFMAs alone then take ~8 us per tile and the loads of operands that the
+// previous kernel just wrote come from memory.  With nsplit > 1, blockIdx.z =
+// molecule * nsplit + split: each split walks its own range of K steps and
+// stores its partial tile, and k_bgemm_reduce (the next launch, so no fences
+// between workgroups) adds the partials in split order -- the same sum
+// however the splits were scheduled -- and applies alpha and beta.
 typedef double qx_sv4 __attribute__((ext_vector_type(4)));
 constexpr int BM = 64, BN = 64, BK = 32, GT = 256, PAD = 2;
 constexpr int KSPLIT_MAX = 8;
 constexpr int PART_TILE = BM * BN;  // doubles of one partial tile (16 per thread)
 
+// Whether the GEMM of molecule b skips its work (molecule done, or its
+// purification already converged for purification step parity pslot).
+__device__ __forceinline__ bool gemm_skips(const double* sc, int b, int skip, int pslot) {
+  return (skip && mol_done(sc, b)) || (pslot >= 0 && sc[b * QS_N + QS_PCONV0 + pslot] != 0.0);
+}
+
 template <bool TA, bool TB>
 __global__ __launch_bounds__(GT) void k_bgemm(int nm, const int* nb, const double* __restrict__ A,
                                               const double* __restrict__ B, double* __restrict__ C, double alpha,
                                               double beta, const double* sc, int skip, int pslot, int nsplit,
-                                              double* part, unsigned* cnt) {
+                                              double* __restrict__ part) {
   const int b = (int)blockIdx.z / nsplit, split = (int)blockIdx.z % nsplit;
-  if (skip && mol_done(sc, b)) return;
-  if (pslot >= 0 && sc[b * QS_N + QS_PCONV0 + pslot] != 0.0) return;  // purification already converged
+  if (gemm_skips(sc, b, skip, pslot)) return;
   const int n = nb[b];
   const long long i0 = (long long)blockIdx.y * BM, j0 = (long long)blockIdx.x * BN;
   if (i0 >= n || j0 >= n) return;  // block-uniform
@@ -167,28 +172,12 @@ __global__ __launch_bounds__(GT) void k_bgemm(int nm, const int* nb, const doubl
     }
     __syncthreads();
   }
-  if (nsplit > 1) {
-    // partial tile out (element q of every thread contiguous across the
-    // block), then the arrival count; the last split adds them all up
+  if (nsplit > 1) {  // partial tile out, element q of every thread contiguous across the block
     const size_t tile = ((size_t)b * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
     double* mine = part + (tile * nsplit + split) * PART_TILE;
 #pragma unroll
     for (int q = 0; q < 16; ++q) mine[q * GT + tid] = acc[q >> 3][(q >> 2) & 1][q & 3];
-    __threadfence();
-    __syncthreads();
-    __shared__ unsigned arrived;
-    if (tid == 0) arrived = atomicAdd(&cnt[tile], 1u);
-    __syncthreads();
-    if (arrived != (unsigned)(nsplit - 1)) return;  // block-uniform
-    __threadfence();
-    const double* all = part + tile * nsplit * PART_TILE;
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      double v = 0.0;
-      for (int t = 0; t < nsplit; ++t) v += all[(size_t)t * PART_TILE + q * GT + tid];
-      acc[q >> 3][(q >> 2) & 1][q & 3] = v;
-    }
-    if (tid == 0) atomicExch(&cnt[tile], 0u);
+    return;
   }
 #pragma unroll
   for (int ta = 0; ta < 2; ++ta)
@@ -203,6 +192,35 @@ __global__ __launch_bounds__(GT) void k_bgemm(int nm, const int* nb, const doubl
           *p = beta == 0.0 ? alpha * acc[ta][tb][r] : alpha * acc[ta][tb][r] + beta * *p;
         }
       }
+}
+
+// C[b] = alpha (sum of the nsplit partial tiles, in split order) + beta C[b];
+// grid (tiles, tiles, molecules), the thread -> element map of k_bgemm.
+__global__ __launch_bounds__(GT) void k_bgemm_reduce(int nm, const int* nb, double* __restrict__ C, double alpha,
+                                                     double beta, const double* sc, int skip, int pslot, int nsplit,
+                                                     const double* __restrict__ part) {
+  const int b = (int)blockIdx.z;
+  if (gemm_skips(sc, b, skip, pslot)) return;
+  const int n = nb[b];
+  const long long i0 = (long long)blockIdx.y * BM, j0 = (long long)blockIdx.x * BN;
+  if (i0 >= n || j0 >= n) return;
+  C += (size_t)b * nm * nm;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = wave >> 1, wc = wave & 1;
+  const size_t tile = ((size_t)b * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+  const double* all = part + tile * nsplit * PART_TILE;
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int ta = q >> 3, tb = (q >> 2) & 1, r = q & 3;
+    double v = 0.0;
+    for (int t = 0; t < nsplit; ++t) v += all[(size_t)t * PART_TILE + q * GT + tid];
+    const long long gi = i0 + wr * 32 + ta * 16 + 4 * r + (lane >> 4);
+    const long long gj = j0 + wc * 32 + tb * 16 + (lane & 15);
+    if (gi < n && gj < n) {
+      double* p = &C[(size_t)gi * nm + gj];
+      *p = beta == 0.0 ? alpha * v : alpha * v + beta * *p;
+    }
+  }
 }
 
 // ---- elementwise / reduction kernels --------------------------------------
@@ -517,10 +535,9 @@ struct qx_scf {
   // ntot x ntot
   double *Dfull = nullptr, *Jfull = nullptr, *Kfull = nullptr;
   double* sc = nullptr;
-  // split-K GEMMs (few tiles per launch): splits, partial tiles, arrival counters
+  // split-K GEMMs (few tiles per launch): splits and partial tiles
   int ksplit = 1;
   double* gpart = nullptr;
-  unsigned* gcnt = nullptr;
   std::vector<void*> bufs;
   bool initialised = false;
   hipGraph_t graph = nullptr;
@@ -583,11 +600,13 @@ void gemm(qx_scf* s, bool ta, bool tb, const double* A, const double* B, double*
   const dim3 grid((unsigned)((s->nm + BN - 1) / BN), (unsigned)((s->nm + BM - 1) / BM), (unsigned)(s->nmol * ns));
   const int ps = pslot < 0 ? -1 : (pslot & 1);
   double* pt = s->gpart;
-  unsigned* ct = s->gcnt;
-  if (!ta && !tb) hipLaunchKernelGGL((k_bgemm<false, false>), grid, dim3(GT), 0, st, s->nm, s->nb, A, B, C, alpha, beta, s->sc, 1, ps, ns, pt, ct);
-  else if (ta && !tb) hipLaunchKernelGGL((k_bgemm<true, false>), grid, dim3(GT), 0, st, s->nm, s->nb, A, B, C, alpha, beta, s->sc, 1, ps, ns, pt, ct);
-  else if (!ta && tb) hipLaunchKernelGGL((k_bgemm<false, true>), grid, dim3(GT), 0, st, s->nm, s->nb, A, B, C, alpha, beta, s->sc, 1, ps, ns, pt, ct);
-  else hipLaunchKernelGGL((k_bgemm<true, true>), grid, dim3(GT), 0, st, s->nm, s->nb, A, B, C, alpha, beta, s->sc, 1, ps, ns, pt, ct);
+  if (!ta && !tb) hipLaunchKernelGGL((k_bgemm<false, false>), grid, dim3(GT), 0, st, s->nm, s->nb, A, B, C, alpha, beta, s->sc, 1, ps, ns, pt);
+  else if (ta && !tb) hipLaunchKernelGGL((k_bgemm<true, false>), grid, dim3(GT), 0, st, s->nm, s->nb, A, B, C, alpha, beta, s->sc, 1, ps, ns, pt);
+  else if (!ta && tb) hipLaunchKernelGGL((k_bgemm<false, true>), grid, dim3(GT), 0, st, s->nm, s->nb, A, B, C, alpha, beta, s->sc, 1, ps, ns, pt);
+  else hipLaunchKernelGGL((k_bgemm<true, true>), grid, dim3(GT), 0, st, s->nm, s->nb, A, B, C, alpha, beta, s->sc, 1, ps, ns, pt);
+  if (ns > 1)
+    hipLaunchKernelGGL(k_bgemm_reduce, dim3(grid.x, grid.y, (unsigned)s->nmol), dim3(GT), 0, st, s->nm, s->nb, C, alpha,
+                       beta, s->sc, 1, ps, ns, pt);
   check(hipGetLastError(), "scf gemm");
 }
 
@@ -730,7 +749,7 @@ int qx_scf_create(qx_ctx* ctx, int nmol, int nm, const int* nbf, const int* offs
       QS_HIP(hipMemset(p, 0, nb2 * sizeof(double)));
     // split K when one launch has too few tiles to spread over the CUs
     // (about 256 workgroups wanted): at most KSPLIT_MAX splits, each with at
-    // least one K step; the arrival counters start (and are left) at zero
+    // least one K step
     {
       const int tiles = (nm + BM - 1) / BM, nk = (nm + BK - 1) / BK;
       const long long wg = (long long)tiles * tiles * nmol;
@@ -738,11 +757,7 @@ int qx_scf_create(qx_ctx* ctx, int nmol, int nm, const int* nbf, const int* offs
       ns = std::min(ns, nk);
       const int per = (nk + ns - 1) / ns;
       s->ksplit = (nk + per - 1) / per;  // no empty split
-      if (s->ksplit > 1) {
-        s->gpart = salloc<double>(s, (size_t)wg * s->ksplit * PART_TILE);
-        s->gcnt = salloc<unsigned>(s, (size_t)wg);
-        QS_HIP(hipMemset(s->gcnt, 0, (size_t)wg * sizeof(unsigned)));
-      }
+      if (s->ksplit > 1) s->gpart = salloc<double>(s, (size_t)wg * s->ksplit * PART_TILE);
     }
     QS_HIP(hipMemset(s->Fh, 0, nb2 * s->m * sizeof(double)));
     QS_HIP(hipMemset(s->Eh, 0, nb2 * s->m * sizeof(double)));

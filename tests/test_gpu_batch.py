@@ -216,9 +216,9 @@ def test_gradient_window_paths_agree(gpu):
     """The adjoint kernels accumulate a block's gradient terms in LDS over
     the atom window its quartets touch, or in global memory when the window
     is wider than the limit: a limit of 1 or 2 atoms sends all or some
-    blocks down the global path, and the gradient (and the forward-over-
-    reverse directional derivative, through the tangent kernels) must not
-    change beyond summation order."""
+    blocks down the global path, and the gradient must not change beyond
+    summation order (nor the forward-over-reverse directional derivative,
+    whose tangent kernels accumulate all 3N slots and ignore the limit)."""
     from test_gpu_eri import small_d_cluster
     from quax_amd.engine import EriEngine
     mol, basis = small_d_cluster()
