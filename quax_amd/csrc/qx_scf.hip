@@ -209,16 +209,28 @@ __global__ __launch_bounds__(GT) void k_bgemm_reduce(int nm, const int* nb, doub
   const int wr = wave >> 1, wc = wave & 1;
   const size_t tile = ((size_t)b * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
   const double* all = part + tile * nsplit * PART_TILE;
+  // all loads of a split issued together (a thread's 16 x nsplit loads would
+  // otherwise run one latency after another)
+  double v[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) v[q] = 0.0;
+#pragma unroll
+  for (int t = 0; t < KSPLIT_MAX; ++t)
+    if (t < nsplit) {
+      double x[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) x[q] = all[(size_t)t * PART_TILE + q * GT + tid];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) v[q] += x[q];
+    }
 #pragma unroll
   for (int q = 0; q < 16; ++q) {
     const int ta = q >> 3, tb = (q >> 2) & 1, r = q & 3;
-    double v = 0.0;
-    for (int t = 0; t < nsplit; ++t) v += all[(size_t)t * PART_TILE + q * GT + tid];
     const long long gi = i0 + wr * 32 + ta * 16 + 4 * r + (lane >> 4);
     const long long gj = j0 + wc * 32 + tb * 16 + (lane & 15);
     if (gi < n && gj < n) {
       double* p = &C[(size_t)gi * nm + gj];
-      *p = beta == 0.0 ? alpha * v : alpha * v + beta * *p;
+      *p = beta == 0.0 ? alpha * v[q] : alpha * v[q] + beta * *p;
     }
   }
 }
