@@ -65,6 +65,23 @@ constexpr double PURIFY_TOL = 1e-8;
 constexpr int MCWEENY_STEPS = 2;
 
 constexpr int SB = 256;  // threads of the reduction and elementwise kernels
+// threads of the per-molecule reductions (energy, DIIS dots, Gershgorin
+// bounds): one block per molecule, so a block must keep many loads in flight
+constexpr int RB = 1024;
+
+// Sum over the RB threads of a block (sh: RB doubles of LDS), in a fixed
+// order; every thread gets the result.
+__device__ __forceinline__ double block_sum_rb(double v, double* sh) {
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = RB / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  const double r = sh[0];
+  __syncthreads();
+  return r;
+}
 
 __device__ __forceinline__ double block_sum(double v, double* sh) {
   sh[threadIdx.x] = v;
@@ -292,21 +309,28 @@ __global__ void k_shift(double* Fp, int nm, const int* nb, const double* sc) {
 
 // energy sum (F + H) D + E_nuc and, when err != null, the RMS of err over
 // the molecule's n x n block; grid (nmol), block SB
-__global__ __launch_bounds__(SB) void k_energy(const double* F, const double* H, const double* D, const double* err,
+__global__ __launch_bounds__(RB) void k_energy(const double* F, const double* H, const double* D, const double* err,
                                                int nm, const int* nb, double* sc) {
-  __shared__ double sh[SB];
+  __shared__ double sh[RB];
   const int b = blockIdx.x;
   if (mol_done(sc, b)) return;
   const int n = nb[b];
   const size_t base = (size_t)b * nm * nm;
+  const long long nn = (long long)n * n;
   double se = 0.0, sr = 0.0;
-  for (long long e = threadIdx.x; e < (long long)n * n; e += SB) {
-    const size_t o = base + (size_t)(e / n) * nm + (size_t)(e % n);
-    se += (F[o] + H[o]) * D[o];
-    if (err) sr += err[o] * err[o];
+  for (long long e0 = threadIdx.x; e0 < nn; e0 += 4 * RB) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const long long e = e0 + (long long)u * RB;
+      if (e < nn) {
+        const size_t o = base + (size_t)(e / n) * nm + (size_t)(e % n);
+        se += (F[o] + H[o]) * D[o];
+        if (err) sr += err[o] * err[o];
+      }
+    }
   }
-  se = block_sum(se, sh);
-  sr = block_sum(sr, sh);
+  se = block_sum_rb(se, sh);
+  sr = block_sum_rb(sr, sh);
   if (threadIdx.x == 0) {
     sc[b * QS_N + QS_ENEW] = se + sc[b * QS_N + QS_ENUC];
     if (err) sc[b * QS_N + QS_DRMS] = sqrt(sr / ((double)n * n));
@@ -328,9 +352,9 @@ __global__ void k_diis_store(const double* F, const double* err, double* Fh, dou
 }
 
 // grid (m, nmol), block SB: Bm[slot][jj] = Bm[jj][slot] = <E_slot, E_jj>
-__global__ __launch_bounds__(SB) void k_diis_dots(const double* Eh, double* Bm, int nm, int m, const int* nb,
+__global__ __launch_bounds__(RB) void k_diis_dots(const double* Eh, double* Bm, int nm, int m, const int* nb,
                                                   const double* sc) {
-  __shared__ double sh[SB];
+  __shared__ double sh[RB];
   const int b = blockIdx.y, jj = blockIdx.x;
   if (mol_done(sc, b)) return;
   const int it = (int)sc[b * QS_N + QS_IT];
@@ -339,12 +363,19 @@ __global__ __launch_bounds__(SB) void k_diis_dots(const double* Eh, double* Bm, 
   const int n = nb[b];
   const double* x = Eh + ((size_t)b * m + slot) * nm * nm;
   const double* y = Eh + ((size_t)b * m + jj) * nm * nm;
+  const long long nn = (long long)n * n;
   double s = 0.0;
-  for (long long e = threadIdx.x; e < (long long)n * n; e += SB) {
-    const size_t o = (size_t)(e / n) * nm + (size_t)(e % n);
-    s += x[o] * y[o];
+  for (long long e0 = threadIdx.x; e0 < nn; e0 += 4 * RB) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const long long e = e0 + (long long)u * RB;
+      if (e < nn) {
+        const size_t o = (size_t)(e / n) * nm + (size_t)(e % n);
+        s += x[o] * y[o];
+      }
+    }
   }
-  s = block_sum(s, sh);
+  s = block_sum_rb(s, sh);
   if (threadIdx.x == 0) {
     Bm[((size_t)b * m + slot) * m + jj] = s;
     Bm[((size_t)b * m + jj) * m + slot] = s;
@@ -356,46 +387,70 @@ __global__ __launch_bounds__(SB) void k_diis_dots(const double* Eh, double* Bm, 
 // augmented matrix in LDS); fewer than two vectors or a zero pivot:
 // c = e_slot (no extrapolation).  grid (nmol), block 64
 constexpr int DIIS_MAX = 32;
-__global__ void k_diis_solve(const double* Bm, double* coef, int m, const double* sc) {
+// Pulay system of molecule b by Gaussian elimination with partial pivoting
+// in LDS: one wavefront, the row updates of each pivot step spread over its
+// lanes (each row's arithmetic as a serial elimination would do it), pivot
+// choice and back substitution by lane 0.
+__global__ __launch_bounds__(64) void k_diis_solve(const double* Bm, double* coef, int m, const double* sc) {
   const int b = blockIdx.x;
-  if (threadIdx.x != 0 || mol_done(sc, b)) return;
+  if (mol_done(sc, b)) return;
+  const int tid = threadIdx.x;
   const int it = (int)sc[b * QS_N + QS_IT];
   const int count = std::min(it + 1, m), slot = it % m;
   double* c = coef + (size_t)b * m;
-  for (int k = 0; k < m; ++k) c[k] = 0.0;
+  __shared__ double a[DIIS_MAX + 1][DIIS_MAX + 2];
+  __shared__ int piv, singular;
+  for (int k = tid; k < m; k += 64) c[k] = 0.0;
+  __syncthreads();
   if (count < 2) {
-    c[slot] = 1.0;
+    if (tid == 0) c[slot] = 1.0;
     return;
   }
   const int q = count + 1;
-  __shared__ double a[DIIS_MAX + 1][DIIS_MAX + 2];
-  for (int r = 0; r < q; ++r) {
-    for (int k = 0; k < q; ++k)
+  for (int idx = tid; idx < q * (q + 1); idx += 64) {
+    const int r = idx / (q + 1), k = idx % (q + 1);
+    if (k < q)
       a[r][k] = (r < count && k < count) ? Bm[((size_t)b * m + r) * m + k] : (r == count && k == count ? 0.0 : -1.0);
-    a[r][q] = r == count ? -1.0 : 0.0;
+    else
+      a[r][q] = r == count ? -1.0 : 0.0;
   }
+  __syncthreads();
   for (int k = 0; k < q; ++k) {
-    int p = k;
-    for (int r = k + 1; r < q; ++r)
-      if (fabs(a[r][k]) > fabs(a[p][k])) p = r;
-    if (a[p][k] == 0.0) {
-      c[slot] = 1.0;
+    if (tid == 0) {
+      int p = k;
+      for (int r = k + 1; r < q; ++r)
+        if (fabs(a[r][k]) > fabs(a[p][k])) p = r;
+      piv = p;
+      singular = a[p][k] == 0.0;
+    }
+    __syncthreads();
+    if (singular) {  // block-uniform
+      if (tid == 0) c[slot] = 1.0;
       return;
     }
+    const int p = piv;
     if (p != k)
-      for (int t = 0; t <= q; ++t) { const double v = a[k][t]; a[k][t] = a[p][t]; a[p][t] = v; }
-    for (int r = k + 1; r < q; ++r) {
+      for (int t = tid; t <= q; t += 64) {
+        const double v = a[k][t];
+        a[k][t] = a[p][t];
+        a[p][t] = v;
+      }
+    __syncthreads();
+    for (int r = k + 1 + tid; r < q; r += 64) {
       const double f = a[r][k] / a[k][k];
       for (int t = k; t <= q; ++t) a[r][t] -= f * a[k][t];
     }
+    __syncthreads();
   }
-  // back substitution in place: column q becomes the solution
-  for (int r = q - 1; r >= 0; --r) {
-    double v = a[r][q];
-    for (int t = r + 1; t < q; ++t) v -= a[r][t] * a[t][q];
-    a[r][q] = v / a[r][r];
+  if (tid == 0) {
+    // back substitution in place: column q becomes the solution
+    for (int r = q - 1; r >= 0; --r) {
+      double v = a[r][q];
+      for (int t = r + 1; t < q; ++t) v -= a[r][t] * a[t][q];
+      a[r][q] = v / a[r][r];
+    }
+    for (int k = 0; k < count; ++k) c[k] = a[k][q];
   }
-  for (int k = 0; k < count; ++k) c[k] = a[k][q];
 }
 
 __global__ void k_diis_combine(const double* Fh, const double* coef, double* Fe, int nm, int m, const double* sc) {
@@ -409,33 +464,44 @@ __global__ void k_diis_combine(const double* Fh, const double* coef, double* Fe,
 
 // ---- purification ----------------------------------------------------------
 // Gershgorin bounds of F' (grid nmol, block SB)
-__global__ __launch_bounds__(SB) void k_bounds(const double* Fp, int nm, const int* nb, double* sc) {
-  __shared__ double lo[SB], hi[SB];
+__global__ __launch_bounds__(RB) void k_bounds(const double* Fp, int nm, const int* nb, double* sc) {
+  __shared__ double lo[RB / 64], hi[RB / 64];
   const int b = blockIdx.x;
   if (mol_done(sc, b)) return;
   const int n = nb[b];
   const double* F = Fp + (size_t)b * nm * nm;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   double mn = 1e300, mx = -1e300;
-  for (int i = threadIdx.x; i < n; i += SB) {
+  // a wavefront per row: lanes over the columns, four loads in flight each
+  for (int i = wave; i < n; i += RB / 64) {
+    const double* row = F + (size_t)i * nm;
     double r = 0.0;
-    for (int j = 0; j < n; ++j)
-      if (j != i) r += fabs(F[(size_t)i * nm + j]);
-    mn = fmin(mn, F[(size_t)i * nm + i] - r);
-    mx = fmax(mx, F[(size_t)i * nm + i] + r);
-  }
-  lo[threadIdx.x] = mn;
-  hi[threadIdx.x] = mx;
-  __syncthreads();
-  for (int o = SB / 2; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) {
-      lo[threadIdx.x] = fmin(lo[threadIdx.x], lo[threadIdx.x + o]);
-      hi[threadIdx.x] = fmax(hi[threadIdx.x], hi[threadIdx.x + o]);
+    for (int j0 = lane; j0 < n; j0 += 4 * 64) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int j = j0 + u * 64;
+        if (j < n && j != i) r += fabs(row[j]);
+      }
     }
-    __syncthreads();
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) r += __shfl_xor(r, off);
+    const double d = row[i];
+    mn = fmin(mn, d - r);
+    mx = fmax(mx, d + r);
   }
+  if (lane == 0) {
+    lo[wave] = mn;
+    hi[wave] = mx;
+  }
+  __syncthreads();
   if (threadIdx.x == 0) {
-    sc[b * QS_N + QS_EMIN] = lo[0];
-    sc[b * QS_N + QS_EMAX] = hi[0];
+    double a = lo[0], z = hi[0];
+    for (int w = 1; w < RB / 64; ++w) {
+      a = fmin(a, lo[w]);
+      z = fmax(z, hi[w]);
+    }
+    sc[b * QS_N + QS_EMIN] = a;
+    sc[b * QS_N + QS_EMAX] = z;
   }
 }
 
@@ -642,7 +708,7 @@ void density(qx_scf* s, const double* Fsrc, hipStream_t st) {
     hipLaunchKernelGGL(k_shift, dim3((unsigned)((s->nm + SB - 1) / SB), (unsigned)s->nmol), dim3(SB), 0, st, s->T2,
                        s->nm, s->nb, s->sc);
   }
-  hipLaunchKernelGGL(k_bounds, dim3((unsigned)s->nmol), dim3(SB), 0, st, s->T2, s->nm, s->nb, s->sc);
+  hipLaunchKernelGGL(k_bounds, dim3((unsigned)s->nmol), dim3(RB), 0, st, s->T2, s->nm, s->nb, s->sc);
   hipLaunchKernelGGL(k_tc2_init, egrid(s), dim3(SB), 0, st, s->T2, s->X, s->nm, s->nb, s->sc);
   check(hipGetLastError(), "scf purification setup");
   double* cur = s->X;
@@ -677,13 +743,13 @@ void iteration(qx_scf* s, hipStream_t st) {
   hipLaunchKernelGGL(k_antisym, egrid(s), dim3(SB), 0, st, s->T2, s->T1, s->nm, s->sc);
   gemm(s, false, false, s->T1, s->A, s->T2, 1.0, 0.0, st);   // e A
   gemm(s, false, false, s->A, s->T2, s->err, 1.0, 0.0, st);  // A e A
-  hipLaunchKernelGGL(k_energy, dim3((unsigned)s->nmol), dim3(SB), 0, st, s->F, s->H, s->D, s->err, s->nm, s->nb,
+  hipLaunchKernelGGL(k_energy, dim3((unsigned)s->nmol), dim3(RB), 0, st, s->F, s->H, s->D, s->err, s->nm, s->nb,
                      s->sc);
   check(hipGetLastError(), "scf energy");
   const double* Fuse = s->F;
   if (s->diis) {
     hipLaunchKernelGGL(k_diis_store, egrid(s), dim3(SB), 0, st, s->F, s->err, s->Fh, s->Eh, s->nm, s->m, s->sc);
-    hipLaunchKernelGGL(k_diis_dots, dim3((unsigned)s->m, (unsigned)s->nmol), dim3(SB), 0, st, s->Eh, s->Bm, s->nm,
+    hipLaunchKernelGGL(k_diis_dots, dim3((unsigned)s->m, (unsigned)s->nmol), dim3(RB), 0, st, s->Eh, s->Bm, s->nm,
                        s->m, s->nb, s->sc);
     hipLaunchKernelGGL(k_diis_solve, dim3((unsigned)s->nmol), dim3(64), 0, st, s->Bm, s->coef, s->m, s->sc);
     hipLaunchKernelGGL(k_diis_combine, egrid(s), dim3(SB), 0, st, s->Fh, s->coef, s->Fe, s->nm, s->m, s->sc);
@@ -811,7 +877,7 @@ int qx_scf_init(qx_scf* s, const double* D0, void* stream) {
     QS_HIP(hipMemcpyAsync(s->D, D0, nb2 * sizeof(double), hipMemcpyDeviceToDevice, st));
     jk(s, s->D, st);
     hipLaunchKernelGGL(k_fock, egrid(s), dim3(SB), 0, st, s->H, s->J, s->K, s->F, s->nm, s->sc);
-    hipLaunchKernelGGL(k_energy, dim3((unsigned)s->nmol), dim3(SB), 0, st, s->F, s->H, s->D, (const double*)nullptr,
+    hipLaunchKernelGGL(k_energy, dim3((unsigned)s->nmol), dim3(RB), 0, st, s->F, s->H, s->D, (const double*)nullptr,
                        s->nm, s->nb, s->sc);
     check(hipGetLastError(), "scf init");
     density(s, s->F, st);
