@@ -278,3 +278,23 @@ def test_tuned_tables_reach_the_generated_class_table(tmp_path):
     for nm, t, f32 in zip(names, tiles, fp32):
         assert t == (tuning["tiles"]["jk"].get(nm, 128), tuning["tiles"]["grad"].get(nm, 128)), nm
         assert f32 == (int(nm in tuning["fp32"]["jk"]), int(nm in tuning["fp32"]["grad"])), nm
+
+
+def test_streaming_kernels_fit_side_stream_scratch():
+    """Every streaming J/K and adjoint-gradient kernel of the last build needs
+    at most 6 KB of scratch per lane (the side streams' budget; read from the
+    compiled code objects), and the reader finds the kernels' sizes."""
+    import glob
+    from quax_amd.csrc import build
+    objs = {}
+    for f in glob.glob(os.path.join(build.OBJ, "cls_*_[jg]*.hip.*.o")):
+        src = os.path.basename(f).split(".hip.")[0]
+        if src not in objs or os.path.getmtime(f) > os.path.getmtime(objs[src]):
+            objs[src] = f
+    if not objs or not os.path.exists(os.path.join(build.LLVM_BIN, "clang-offload-bundler")):
+        pytest.skip("no built objects or no offload bundler")
+    import tempfile
+    with tempfile.TemporaryDirectory() as tmp:
+        sizes = build.kernel_scratch(objs["cls_2221_j0"], tmp)
+    assert sizes and all(name.startswith("_Z") and size >= 0 for name, size in sizes)
+    build.check_streaming_scratch(list(objs.values()), verbose=False)
