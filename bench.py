@@ -160,6 +160,23 @@ def class_times(table, ms, steps):
     return out
 
 
+def class_roofline(eng, model, times):
+    """Per class, one stream: quartets of the J/K and adjoint passes,
+    algorithmic GFLOP per pass (the FLOP model's per-quartet counts) and the
+    TFLOP/s its FP64 kernels reach."""
+    out = {}
+    for l4, ne, nd, _ in eng.class_quartets():
+        nm = "".join(map(str, l4))
+        t = times.get(nm, {})
+        jk_ms, gr_ms = t.get("jk64", 0.0), t.get("deriv64", 0.0)
+        gj, gg = model[nm]["jk"] * ne / 1e9, model[nm]["adjoint"] * nd / 1e9
+        out[nm] = {"jk_quartets": int(ne), "grad_quartets": int(nd), "jk_gflop": gj, "grad_gflop": gg,
+                   "jk_ms": jk_ms, "grad_ms": gr_ms,
+                   "jk_tflops": gj / jk_ms if jk_ms > 0 else None,
+                   "grad_tflops": gg / gr_ms if gr_ms > 0 else None}
+    return out
+
+
 def cpu_baseline(mol, basis, tol, eng, seconds):
     """Times the CPU oracle (C restatement of the reference's quartet loop
     and of the Libint HGP arithmetic) on a seeded uniform sample of the same
@@ -603,6 +620,7 @@ def main():
         "hf_cluster24": hf24,
         "top_kernels_ms_per_step": kernels,
         "class_kernel_ms_per_step": class_times(table, ms, args.steps),
+        "class_roofline": class_roofline(eng, model, class_times(table, ms, args.steps)),
     }
     print(json.dumps(out), flush=True)
     eng.close()
