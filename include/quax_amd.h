@@ -287,6 +287,12 @@ int qx_get_grad_wave_reduce(const qx_ctx* ctx, int* flags);
  * copies); the copies are summed once per pass. */
 int qx_set_grad_copies(qx_ctx* ctx, const int* copies);
 int qx_get_grad_copies(const qx_ctx* ctx, int* copies);
+/* Per class (flags[qx_num_classes()], NULL = all on): whether its J/K
+ * kernels stage the K' rows of a block's bra pair in LDS and flush them once
+ * per block (where the kernel's rows fit and the context's row width allows
+ * it), or add each K' term to global memory directly. */
+int qx_set_jk_rows(qx_ctx* ctx, const int* flags);
+int qx_get_jk_rows(const qx_ctx* ctx, int* flags);
 /* Widest atom window (0..256 atoms; 0 = 256) of the adjoint-gradient
  * kernels' LDS accumulators.  A block accumulates its quartets' gradient
  * terms in LDS over the atoms from the lowest to the highest it touches when

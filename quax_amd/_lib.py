@@ -80,6 +80,8 @@ SIGNATURES = {
     "qx_set_grad_wave_reduce": (_i, [_vp, _vp]),
     "qx_set_grad_copies": (_i, [_vp, _vp]),
     "qx_get_grad_copies": (_i, [_vp, _vp]),
+    "qx_set_jk_rows": (_i, [_vp, _vp]),
+    "qx_get_jk_rows": (_i, [_vp, _vp]),
     "qx_set_grad_window": (_i, [_vp, _i]),
     "qx_get_grad_window": (_i, [_vp, _vp]),
     "qx_get_grad_wave_reduce": (_i, [_vp, _vp]),

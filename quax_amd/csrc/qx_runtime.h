@@ -127,6 +127,9 @@ struct qx_ctx {
   // per class: copies of the gradient accumulators its blocks spread their
   // flushes over (1..QX_GRAD_COPIES)
   int gcopy[QX_NUM_CLASSES_MAX] = {};
+  // per class: the J/K kernels stage K' rows in LDS (when the kernel and
+  // the context allow it at all)
+  unsigned char jkrows[QX_NUM_CLASSES_MAX] = {};
   // widest atom window of the adjoint kernels' LDS gradient accumulators,
   // in doubles (0: QX_GRAD_LDS)
   int grad_win = 0;

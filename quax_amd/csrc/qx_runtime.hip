@@ -750,6 +750,7 @@ static int init_impl(int natom, const double* xyz_bohr, const double* nuclear_ch
           c->fp32_ok[pass][k] = qx_fp32_default[k][pass];
           c->tile[pass][k] = qx_tile_default[k][pass];
           c->gwred[k] = qx_gwred_default[k];
+          c->jkrows[k] = 1;
           c->gcopy[k] = qx_gcopy_default[k];
         }
       init_body(c, natom, xyz_bohr, nuclear_charges, nshell, shell_l, shell_atom, shell_nprim, exps, coefs, pure,
@@ -984,6 +985,7 @@ static void add_jk_list(qx_ctx* c, const QxWorkHost& wh, const QxEpiDev& e, int 
     QxArgs a;
     a.pr = c->pr; a.pp = c->pp; a.w = wh.w; a.e = e; a.boys = c->d_boys;
     a.w.sel = sel;
+    a.e.krows = e.krows && c->jkrows[wh.cls];
     const int slot = QX_SLOT_JK(wh.cls, k, fp32 ? 1 : 0);
     auto fn = ent.jk[k];
     const int block = c->tile[0][wh.cls];
@@ -1598,6 +1600,18 @@ int qx_set_grad_copies(qx_ctx* c, const int* copies) {
     for (int k = 0; k < QX_NUM_CLASSES; ++k)
       if (copies[k] < 1 || copies[k] > QX_GRAD_COPIES) return QX_EINVAL;
   for (int k = 0; k < QX_NUM_CLASSES; ++k) c->gcopy[k] = copies ? copies[k] : qx_gcopy_default[k];
+  return QX_OK;
+}
+
+int qx_set_jk_rows(qx_ctx* c, const int* flags) {
+  if (!c) return QX_EINVAL;
+  for (int k = 0; k < QX_NUM_CLASSES; ++k) c->jkrows[k] = flags ? (flags[k] != 0) : 1;
+  return QX_OK;
+}
+
+int qx_get_jk_rows(const qx_ctx* c, int* flags) {
+  if (!c || !flags) return QX_EINVAL;
+  for (int k = 0; k < QX_NUM_CLASSES; ++k) flags[k] = c->jkrows[k];
   return QX_OK;
 }
 

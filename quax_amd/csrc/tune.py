@@ -12,6 +12,10 @@ tables (qx_set_class_tiles / qx_set_grad_wave_reduce / qx_set_fp32_classes
 override them at run time):
     python -m quax_amd.csrc.tune --tiles --merge quax_amd/csrc/deriv_budget.json --out tiles.json
 
+--jk-rows (in process): per class, the J/K kernels' time with K' rows staged
+in LDS against every K' term added to global memory (a measurement only):
+    python -m quax_amd.csrc.tune --jk-rows
+
 Otherwise (several libraries): the node budget at which gen_eri.py splits
 the adjoint-gradient kernel (--key budget) or the waves per SIMD its kernels
 are compiled for (--key waves; a library built with --waves W, W = 0 for the
@@ -212,7 +216,50 @@ def tune_tiles(workload, steps, seed=0):
             "ms_per_step_tiles": {"per_class_one_stream": table, "four_streams": res}}
 
 
+def compare_jk_rows(workload, steps, seed=0):
+    """Per class, J/K kernel ms per pass on one stream with the K' rows of a
+    block's bra pair staged in LDS (qx_set_jk_rows on) and with every K' term
+    added to global memory (off); best of two rounds each."""
+    import numpy as np
+    import torch
+    from quax_amd import workloads
+    from quax_amd.basis import BasisSet
+    from quax_amd.engine import EriEngine
+    mol = getattr(workloads, workload)()
+    bs = BasisSet.build(mol, "even-tempered", pure=True)
+    with EriEngine(mol, bs, ints_tol=1e-12) as eng:
+        eng.set_streams(1)
+        n = eng.nbf
+        X = np.random.default_rng(seed).standard_normal((n, n)) * 0.3
+        D = torch.as_tensor(X @ X.T / n, device=eng.device)
+        nc = eng.L.qx_num_classes()
+        names = _class_names(eng.L)
+        res = {}
+        for _ in range(2):
+            for on in (1, 0):
+                eng.set_jk_rows([on] * nc)
+                t = _pass_times(eng, D, steps)
+                for nm in names:
+                    res.setdefault((nm, on), []).append(t.get((nm, 0), 0.0))
+        eng.set_jk_rows(None)
+        ton = toff = 0.0
+        for nm in names:
+            a, b = min(res[(nm, 1)]), min(res[(nm, 0)])
+            ton += a
+            toff += b
+            print(f"[tune] {nm} J/K: K' rows in LDS {a:.3f} ms, direct {b:.3f} ms ({b / a:.2f}x)", flush=True)
+        print(f"[tune] J/K pass: K' rows in LDS {ton:.2f} ms, direct {toff:.2f} ms", flush=True)
+
+
 def main():
+    if "--jk-rows" in sys.argv:
+        ap = argparse.ArgumentParser()
+        ap.add_argument("--jk-rows", action="store_true")
+        ap.add_argument("--workload", default="cluster10")
+        ap.add_argument("--steps", type=int, default=4)
+        a = ap.parse_args()
+        compare_jk_rows(a.workload, a.steps)
+        return
     if "--tiles" in sys.argv:
         ap = argparse.ArgumentParser()
         ap.add_argument("--tiles", action="store_true")

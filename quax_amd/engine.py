@@ -192,6 +192,22 @@ class EriEngine:
             raise ValueError("one copy count per kernel class")
         check(self.L.qx_set_grad_copies(self.ctx, f.ctypes.data_as(ctypes.c_void_p)), self.ctx, "set_grad_copies")
 
+    def jk_rows(self):
+        """Per kernel class, whether its J/K kernels stage K' rows in LDS."""
+        f = np.zeros(self.L.qx_num_classes(), dtype=np.int32)
+        check(self.L.qx_get_jk_rows(self.ctx, f.ctypes.data_as(ctypes.c_void_p)), self.ctx)
+        return f
+
+    def set_jk_rows(self, flags=None):
+        """Sets them per class (None: all on)."""
+        if flags is None:
+            check(self.L.qx_set_jk_rows(self.ctx, None), self.ctx)
+            return
+        f = np.ascontiguousarray(np.asarray(flags, dtype=np.int32))
+        if f.shape != (self.L.qx_num_classes(),):
+            raise ValueError("one flag per kernel class")
+        check(self.L.qx_set_jk_rows(self.ctx, f.ctypes.data_as(ctypes.c_void_p)), self.ctx, "set_jk_rows")
+
     def grad_window(self):
         """Widest atom window of the adjoint kernels' LDS gradient accumulators."""
         a = ctypes.c_int(0)

@@ -78,3 +78,25 @@ def test_oei_deriv_and_grad_vs_oracle(gpu, maker):
     for a, b in ((dS, dS0), (dT, dT0), (dV, dV0)):
         assert np.abs(a - b).max() < 1e-11
     assert np.abs(g - ref).max() < 1e-10
+
+
+@pytest.mark.parametrize("maker", [small_d_cluster, contracted_d_cluster])
+def test_jk_rows_do_not_change_results(gpu, maker):
+    """K' rows staged in LDS (the default) or every K' term added to global
+    memory (qx_set_jk_rows off): the same J and K up to summation order."""
+    from quax_amd.engine import EriEngine
+    mol, basis = maker()
+    rng = np.random.default_rng(3)
+    n = basis.nbf()
+    X = rng.standard_normal((n, n)) * 0.3
+    D = X @ X.T / n
+    with EriEngine(mol, basis, ints_tol=1e-14) as eng:
+        nc = eng.L.qx_num_classes()
+        assert eng.jk_rows().tolist() == [1] * nc
+        J1, K1 = (x.cpu().numpy() for x in eng.jk(D))
+        eng.set_jk_rows([0] * nc)
+        assert eng.jk_rows().tolist() == [0] * nc
+        J0, K0 = (x.cpu().numpy() for x in eng.jk(D))
+        eng.set_jk_rows(None)
+    assert np.abs(J1 - J0).max() <= 1e-13 * max(1.0, np.abs(J1).max())
+    assert np.abs(K1 - K0).max() <= 1e-13 * max(1.0, np.abs(K1).max())
